@@ -1,0 +1,30 @@
+// gcl_device.h -- host-side interface of the device engine (internal).
+#pragma once
+
+#include "gcl_internal.h"
+
+namespace gcl {
+
+// Per-graph result of a device launch.
+struct DeviceGraphResult {
+    uint64_t nodes;
+    uint64_t automorphisms;
+    uint64_t splitter_steps;
+    uint64_t csr_bytes;
+    int32_t status;   // 0 ok, otherwise a gcl_status
+    int32_t pad;
+};
+
+// Canonical labeling of every graph of a flat batch on device 0.
+//   out_label: vtx_off[G] ints (label[i] = vertex at canonical position i)
+//   results:   G entries or NULL
+//   ring_cap:  automorphism slots per worker; the search runs with
+//              min(max_aut, ring_cap) slots (the canonical form does not
+//              depend on it, only the node count does)
+gcl_status gpu_canon_batch(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t* out_label,
+                           DeviceGraphResult* results, gcl_batch_stats* st);
+
+int32_t device_count();
+uint64_t step_budget();
+
+}  // namespace gcl

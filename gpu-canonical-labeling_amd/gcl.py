@@ -1,0 +1,358 @@
+"""Python binding of libgcl (ctypes over the C ABI of include/gcl.h).
+
+The library is the product: this module only marshals numpy arrays.  It
+raises GclError when libgcl.so has not been built or a call fails; there is no
+Python fallback for any computation.
+
+    import gcl
+    g = gcl.erdos_renyi(128, 0.08, seed=0)
+    res = gcl.canon_label(g, engine="gpu")        # reference --parallel
+    res.keys      # sorted i*n+j keys of the canonical adjacency
+    res.label     # label[i] = input vertex at canonical position i
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgcl.so")
+CLI_PATH = os.path.join(_HERE, "gpu_canon_label")
+
+GCL_OK = 0
+_STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "no device",
+           -5: "i/o error", -6: "capacity exceeded", -7: "internal error", -8: "busy"}
+ENGINES = {"gpu": 0, "parallel": 0, "cpu": 1, "sequential": 1}
+
+
+class GclError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class _CSR(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz", C.c_int64), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("color", C.c_void_p)]
+
+
+class _Graph(C.Structure):
+    _fields_ = [("csr", _CSR), ("row_ptr_buf", C.c_void_p), ("col_buf", C.c_void_p), ("color_buf", C.c_void_p)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("nodes", C.c_uint64), ("automorphisms", C.c_uint64), ("refinements", C.c_uint64),
+                ("splitter_steps", C.c_uint64), ("csr_bytes", C.c_uint64), ("kernel_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
+class _Batch(C.Structure):
+    _fields_ = [("num_graphs", C.c_int32), ("vtx_off", C.c_void_p), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("color", C.c_void_p)]
+
+
+class _BatchStats(C.Structure):
+    _fields_ = [("nodes", C.c_uint64), ("automorphisms", C.c_uint64), ("splitter_steps", C.c_uint64),
+                ("csr_bytes", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("workers", C.c_int32)]
+
+
+_lib = None
+
+# exported symbols and their signatures (restype, argtypes)
+_SIGS = {
+    "gcl_last_error": (C.c_char_p, []),
+    "gcl_status_string": (C.c_char_p, [C.c_int]),
+    "gcl_csr_validate": (C.c_int, [C.c_void_p]),
+    "gcl_nauty_parallel": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "gcl_nauty_sequential": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "gcl_canon_label": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "gcl_is_isomorphic": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "gcl_batch_canon": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gcl_batch_canon_plain": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gcl_canonical_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gcl_read_dimacs": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "gcl_write_dimacs": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "gcl_graph_free": (None, [C.c_void_p]),
+    "gcl_format_adjacency_keys": (C.c_int64, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64]),
+    "gcl_gen_erdos_renyi": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.c_void_p]),
+    "gcl_gen_erdos_renyi_mean_degree": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.c_void_p]),
+    "gcl_gen_random_regular": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p]),
+    "gcl_gen_torus": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p]),
+    "gcl_gen_cfi": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p]),
+    "gcl_gen_permute": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "gcl_gen_edge_swap": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "gcl_device_count": (C.c_int32, []),
+}
+
+
+def lib():
+    """Loads libgcl.so (built by build.py); raises GclError if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GclError(-7, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(status):
+    if status != GCL_OK:
+        raise GclError(status, lib().gcl_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Graph:
+    """Undirected simple graph in CSR form (rows sorted, symmetric)."""
+    n: int
+    row_ptr: np.ndarray
+    col: np.ndarray
+    color: Optional[np.ndarray] = None
+
+    @property
+    def nnz(self):
+        return int(self.row_ptr[self.n]) if self.n >= 0 else 0
+
+    @property
+    def m(self):
+        return self.nnz // 2
+
+    def _csr(self):
+        self.row_ptr = np.ascontiguousarray(self.row_ptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(self.col, dtype=np.int32)
+        if self.color is not None:
+            self.color = np.ascontiguousarray(self.color, dtype=np.int32)
+        c = _CSR(self.n, self.nnz, _ptr(self.row_ptr), _ptr(self.col) if self.col.size else None,
+                 _ptr(self.color))
+        return c
+
+    def edges(self):
+        """Undirected edges (u < v) as an (m, 2) array."""
+        src = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.row_ptr))
+        sel = src < self.col
+        return np.stack([src[sel], self.col[sel].astype(np.int64)], axis=1)
+
+    @staticmethod
+    def from_edges(n, edges, color=None):
+        edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        if edges.size:
+            u = np.concatenate([edges[:, 0], edges[:, 1]])
+            v = np.concatenate([edges[:, 1], edges[:, 0]])
+        else:
+            u = v = np.zeros(0, np.int64)
+        order = np.lexsort((v, u))
+        u, v = u[order], v[order]
+        row_ptr = np.zeros(n + 1, np.int32)
+        np.add.at(row_ptr, u + 1, 1)
+        row_ptr = np.cumsum(row_ptr).astype(np.int32)
+        return Graph(n, row_ptr, v.astype(np.int32), None if color is None else np.asarray(color, np.int32))
+
+
+def _from_owned(gp):
+    g = gp.contents.csr
+    n = g.n
+    rp = np.ctypeslib.as_array(C.cast(g.row_ptr, C.POINTER(C.c_int32)), (n + 1,)).copy()
+    col = (np.ctypeslib.as_array(C.cast(g.col, C.POINTER(C.c_int32)), (g.nnz,)).copy()
+           if g.nnz else np.zeros(0, np.int32))
+    color = (np.ctypeslib.as_array(C.cast(g.color, C.POINTER(C.c_int32)), (n,)).copy()
+             if g.color else None)
+    lib().gcl_graph_free(gp)
+    return Graph(n, rp, col, color)
+
+
+def _gen(name, *args):
+    gp = C.POINTER(_Graph)()
+    _check(getattr(lib(), name)(*args, C.byref(gp)))
+    return _from_owned(gp)
+
+
+def erdos_renyi(n, p, seed):
+    return _gen("gcl_gen_erdos_renyi", n, float(p), seed)
+
+
+def erdos_renyi_mean_degree(n, mean_degree, seed):
+    return _gen("gcl_gen_erdos_renyi_mean_degree", n, float(mean_degree), seed)
+
+
+def random_regular(n, d, seed):
+    return _gen("gcl_gen_random_regular", n, d, seed)
+
+
+def torus(a, b):
+    return _gen("gcl_gen_torus", a, b)
+
+
+def cfi(base_n, twisted, seed):
+    return _gen("gcl_gen_cfi", base_n, 1 if twisted else 0, seed)
+
+
+def permute(g: Graph, seed):
+    c = g._csr()
+    return _gen("gcl_gen_permute", C.byref(c), seed)
+
+
+def edge_swap(g: Graph, seed):
+    c = g._csr()
+    return _gen("gcl_gen_edge_swap", C.byref(c), seed)
+
+
+def read_dimacs(path):
+    gp = C.POINTER(_Graph)()
+    _check(lib().gcl_read_dimacs(os.fsencode(path), C.byref(gp)))
+    return _from_owned(gp)
+
+
+def write_dimacs(path, g: Graph):
+    c = g._csr()
+    _check(lib().gcl_write_dimacs(os.fsencode(path), C.byref(c)))
+
+
+def validate(g: Graph):
+    c = g._csr()
+    _check(lib().gcl_csr_validate(C.byref(c)))
+
+
+@dataclass
+class CanonResult:
+    label: np.ndarray
+    keys: np.ndarray
+    nodes: int
+    automorphisms: int
+    splitter_steps: int
+    csr_bytes: int
+    kernel_ms: float
+    total_ms: float
+
+    def dense(self, n):
+        d = np.zeros(n * n, np.uint8)
+        d[self.keys.astype(np.int64)] = 1
+        return d.reshape(n, n)
+
+
+def canon_label(g: Graph, max_aut=10000, engine="gpu") -> CanonResult:
+    """Canonical labeling (reference Nauty_Parallel / Nauty_Sequential)."""
+    c = g._csr()
+    label = np.zeros(max(g.n, 1), np.int32)
+    keys = np.zeros(max(g.nnz, 1), np.uint32)
+    st = _Stats()
+    _check(lib().gcl_canon_label(C.byref(c), max_aut, ENGINES[engine], _ptr(label), _ptr(keys), None,
+                                 C.byref(st)))
+    return CanonResult(label[:g.n], keys[:g.nnz], st.nodes, st.automorphisms, st.splitter_steps, st.csr_bytes,
+                       st.kernel_ms, st.total_ms)
+
+
+def nauty_parallel(g: Graph, max_aut=10000):
+    """Reference-shaped call: returns (n*n canonical matrix bytes, node count)."""
+    c = g._csr()
+    out = np.zeros(g.n * g.n, np.uint8)
+    nodes = C.c_uint32()
+    _check(lib().gcl_nauty_parallel(C.byref(c), max_aut, _ptr(out), C.byref(nodes)))
+    return out, nodes.value
+
+
+def nauty_sequential(g: Graph, max_aut=10000):
+    c = g._csr()
+    out = np.zeros(g.n * g.n, np.uint8)
+    nodes = C.c_uint32()
+    _check(lib().gcl_nauty_sequential(C.byref(c), max_aut, _ptr(out), C.byref(nodes)))
+    return out, nodes.value
+
+
+def is_isomorphic(a: Graph, b: Graph, max_aut=10000, engine="gpu") -> bool:
+    ca, cb = a._csr(), b._csr()
+    v = C.c_int()
+    _check(lib().gcl_is_isomorphic(C.byref(ca), C.byref(cb), max_aut, ENGINES[engine], C.byref(v)))
+    return bool(v.value)
+
+
+def canonical_keys(g: Graph, label: np.ndarray) -> np.ndarray:
+    c = g._csr()
+    label = np.ascontiguousarray(label, dtype=np.int32)
+    keys = np.zeros(max(g.nnz, 1), np.uint32)
+    _check(lib().gcl_canonical_keys(C.byref(c), _ptr(label), _ptr(keys)))
+    return keys[:g.nnz]
+
+
+def format_adjacency(keys: np.ndarray, n: int) -> str:
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    ln = lib().gcl_format_adjacency_keys(_ptr(keys), keys.size, n, None, 0)
+    buf = C.create_string_buffer(int(ln) + 1)
+    lib().gcl_format_adjacency_keys(_ptr(keys), keys.size, n, buf, ln + 1)
+    return buf.raw[:ln].decode()
+
+
+class Batch:
+    """A flat batch of graphs (the layout of gcl_batch in include/gcl.h)."""
+
+    def __init__(self, graphs: Sequence[Graph]):
+        self.graphs = list(graphs)
+        ns = np.array([g.n for g in self.graphs], np.int64)
+        self.vtx_off = np.zeros(len(ns) + 1, np.int64)
+        self.vtx_off[1:] = np.cumsum(ns)
+        rows, cols, colors = [], [], []
+        base = 0
+        any_color = any(g.color is not None for g in self.graphs)
+        for g in self.graphs:
+            rows.append(g.row_ptr[:-1].astype(np.int64) + base)
+            cols.append(np.asarray(g.col, np.int32))
+            base += g.nnz
+            if any_color:
+                colors.append(g.color if g.color is not None else np.zeros(g.n, np.int32))
+        rows.append(np.array([base], np.int64))
+        self.row_ptr = np.ascontiguousarray(np.concatenate(rows))
+        self.col = np.ascontiguousarray(np.concatenate(cols) if cols else np.zeros(0, np.int32))
+        self.color = np.ascontiguousarray(np.concatenate(colors)) if any_color else None
+
+    def _c(self):
+        return _Batch(len(self.graphs), _ptr(self.vtx_off), _ptr(self.row_ptr),
+                      _ptr(self.col) if self.col.size else None, _ptr(self.color))
+
+    def labels_of(self, labels, i):
+        return labels[self.vtx_off[i]:self.vtx_off[i + 1]]
+
+
+@dataclass
+class BatchResult:
+    labels: np.ndarray
+    nodes: np.ndarray
+    total_nodes: int
+    automorphisms: int
+    splitter_steps: int
+    csr_bytes: int
+    kernel_ms: float
+    total_ms: float
+    workers: int
+
+
+def batch_canon(batch: Batch, max_aut=10000, engine="gpu", plain=False) -> BatchResult:
+    c = batch._c()
+    labels = np.zeros(max(int(batch.vtx_off[-1]), 1), np.int32)
+    nodes = np.zeros(max(len(batch.graphs), 1), np.uint64)
+    st = _BatchStats()
+    if plain:
+        _check(lib().gcl_batch_canon_plain(C.byref(c), max_aut, _ptr(labels), _ptr(nodes), C.byref(st)))
+    else:
+        _check(lib().gcl_batch_canon(C.byref(c), max_aut, ENGINES[engine], _ptr(labels), _ptr(nodes),
+                                     C.byref(st)))
+    return BatchResult(labels[:int(batch.vtx_off[-1])], nodes[:len(batch.graphs)], st.nodes, st.automorphisms,
+                       st.splitter_steps, st.csr_bytes, st.kernel_ms, st.total_ms, st.workers)
+
+
+def device_count():
+    return int(lib().gcl_device_count())

@@ -1,0 +1,203 @@
+/*
+ * gcl.h -- C ABI of libgcl, MI355X-native canonical labeling.
+ *
+ * Drop-in for the canonical-labeling path of rana-dbouk/gpu-canonical-labeling.
+ * Every entry point takes plain pointers and sizes, returns a gcl_status and
+ * never exits or throws across the ABI.  The message of the last failure on
+ * the calling thread is available from gcl_last_error().
+ *
+ * Graph convention (CSRGraph, src/CSRGraph.h:6-39 of the reference): an
+ * undirected simple graph on vertices 0..n-1 stored as a symmetric CSR,
+ * row_ptr[n+1] and col[row_ptr[n]], each row sorted ascending, no self loops,
+ * no repeated neighbours; optional vertex colours (the DIMACS "n" lines).
+ */
+#ifndef GCL_H
+#define GCL_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCL_VERSION_MAJOR 0
+#define GCL_VERSION_MINOR 1
+
+typedef enum gcl_status {
+    GCL_OK = 0,
+    GCL_E_INVALID = -1,   /* malformed graph or argument */
+    GCL_E_NOMEM = -2,     /* host or device allocation failed */
+    GCL_E_DEVICE = -3,    /* HIP runtime error */
+    GCL_E_NODEVICE = -4,  /* no usable GPU */
+    GCL_E_IO = -5,        /* file could not be read or written */
+    GCL_E_CAPACITY = -6,  /* input beyond a documented size limit */
+    GCL_E_INTERNAL = -7,  /* consistency check failed inside the library */
+    GCL_E_BUSY = -8       /* admission refused (server full) */
+} gcl_status;
+
+typedef enum gcl_engine {
+    GCL_ENGINE_GPU = 0,   /* --parallel: HIP search on device 0 */
+    GCL_ENGINE_CPU = 1    /* --sequential: host search, same traversal */
+} gcl_engine;
+
+typedef struct gcl_csr {
+    int32_t n;
+    int64_t nnz;             /* == row_ptr[n] == 2 * edges */
+    const int32_t* row_ptr;  /* n + 1 */
+    const int32_t* col;      /* nnz */
+    const int32_t* color;    /* n, or NULL for a single colour */
+} gcl_csr;
+
+typedef struct gcl_stats {
+    uint64_t nodes;          /* search-tree nodes (root + one per individualize+refine) */
+    uint64_t automorphisms;  /* automorphisms discovered */
+    uint64_t refinements;    /* refinement calls */
+    uint64_t splitter_steps; /* splitter iterations over all refinements */
+    uint64_t csr_bytes;      /* algorithmic CSR bytes read by refinement */
+    double kernel_ms;        /* device time of the search launch(es) */
+    double total_ms;         /* wall time of the call */
+} gcl_stats;
+
+/* ---------------------------------------------------------------- errors */
+const char* gcl_last_error(void);
+const char* gcl_status_string(gcl_status s);
+
+/* ------------------------------------------------------ graph validation */
+/* Checks the CSR convention above (sorted rows, symmetry, no loops, no
+ * repeated neighbours).  Returns GCL_OK or GCL_E_INVALID with a message. */
+gcl_status gcl_csr_validate(const gcl_csr* g);
+
+/* ------------------------------------------- reference-shaped entry points */
+/*
+ * Replaces Nauty_Parallel (reference src/nauty_parallel.h:9-13,
+ * nauty_parallel.cu:640-1031): canonical labeling on the GPU.  out_canon
+ * receives the n*n row-major 0/1 canonical adjacency matrix (the
+ * std::vector<unsigned char> the reference returns); node_count, if not NULL,
+ * receives the search-tree node count.  max_aut must be >= 1.
+ */
+gcl_status gcl_nauty_parallel(const gcl_csr* g, int32_t max_aut,
+                              uint8_t* out_canon, uint32_t* node_count);
+
+/*
+ * Replaces Nauty_Sequential + best_leaf.hash_of_perm_graph (reference
+ * src/sequential.h:14,30; read at main.cpp:133-142): the same search on the
+ * host.  Unlike the reference it does not free the caller's graph.
+ */
+gcl_status gcl_nauty_sequential(const gcl_csr* g, uint32_t max_aut,
+                                uint8_t* out_canon, uint32_t* node_count);
+
+/* ---------------------------------------------------- general entry points */
+/*
+ * Canonical labeling with either engine.
+ *   out_label  n ints or NULL: out_label[i] = input vertex placed at canonical
+ *              position i (the best leaf's element order)
+ *   out_keys   nnz uint32 or NULL: sorted keys i*n+j of the canonical graph's
+ *              directed edges (the sparse form of the canonical adjacency)
+ *   out_dense  n*n bytes or NULL: canonical adjacency matrix
+ *   stats      or NULL
+ */
+gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine,
+                           int32_t* out_label, uint32_t* out_keys,
+                           uint8_t* out_dense, gcl_stats* stats);
+
+/*
+ * Isomorphism verdict of the two-graph CLI mode (main.cpp:173-198):
+ * *verdict = 1 when the canonical forms agree, 0 otherwise.  Graphs with
+ * different n or edge count are reported non-isomorphic (the reference
+ * compares with graph 1's n only).
+ */
+gcl_status gcl_is_isomorphic(const gcl_csr* a, const gcl_csr* b, uint32_t max_aut,
+                             gcl_engine engine, int* verdict);
+
+/* ------------------------------------------------------------- batching */
+/*
+ * A batch of graphs in one flat CSR:
+ *   vtx_off[num_graphs+1]   graph g owns vertices vtx_off[g] .. vtx_off[g+1]-1
+ *   row_ptr[vtx_off[G]+1]   global offsets into col, row_ptr[0] == 0 (row of graph g's local
+ *                           vertex i is row_ptr[vtx_off[g]+i] .. +1)
+ *   col                     LOCAL vertex ids (0 .. n_g-1)
+ *   color                   vtx_off[G] ints or NULL
+ */
+typedef struct gcl_batch {
+    int32_t num_graphs;
+    const int64_t* vtx_off;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const int32_t* color;
+} gcl_batch;
+
+typedef struct gcl_batch_stats {
+    uint64_t nodes;
+    uint64_t automorphisms;
+    uint64_t splitter_steps;
+    uint64_t csr_bytes;
+    double kernel_ms;
+    double total_ms;
+    int32_t workers;         /* concurrent search workers of the launch */
+} gcl_batch_stats;
+
+/*
+ * Canonical labeling of every graph of a batch in one persistent GPU launch
+ * (or, with GCL_ENGINE_CPU, a host loop).  out_label has vtx_off[G] entries
+ * (per graph as in gcl_canon_label); out_nodes (G entries) is optional.
+ * Host pointers; the call copies to the device, runs, and copies back.
+ */
+gcl_status gcl_batch_canon(const gcl_batch* b, uint32_t max_aut, gcl_engine engine,
+                           int32_t* out_label, uint64_t* out_nodes,
+                           gcl_batch_stats* stats);
+
+/* Plain baseline: the same graphs, one graph per launch (gcl_canon_label on
+ * each), for the batched-vs-plain comparison. */
+gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut,
+                                 int32_t* out_label, uint64_t* out_nodes,
+                                 gcl_batch_stats* stats);
+
+/* Canonical keys of one graph from a labeling: out_keys[nnz] = sorted
+ * i*n+j over directed edges of the relabeled graph. */
+gcl_status gcl_canonical_keys(const gcl_csr* g, const int32_t* label, uint32_t* out_keys);
+
+/* --------------------------------------------------------------- DIMACS */
+/* Owned graph returned by the reader (CSRGraph::fromDIMACS,
+ * reference src/CSRGraph.cpp:40-119).  Free with gcl_graph_free. */
+typedef struct gcl_graph {
+    gcl_csr csr;
+    int32_t* row_ptr_buf;
+    int32_t* col_buf;
+    int32_t* color_buf;
+} gcl_graph;
+
+gcl_status gcl_read_dimacs(const char* path, gcl_graph** out);
+gcl_status gcl_write_dimacs(const char* path, const gcl_csr* g);
+void gcl_graph_free(gcl_graph* g);
+
+/*
+ * Canonical adjacency list text of main.cpp:33-70 (print_canonical_adjacency_list):
+ * "%3d : " then " j" per neighbour, wrapped to "\n    " at 80 columns, ";\n"
+ * per row.  Writes at most cap bytes (NUL terminated when room) and returns
+ * the full length, so a first call with cap 0 sizes the buffer.
+ */
+int64_t gcl_format_adjacency_keys(const uint32_t* keys, int64_t nnz, int32_t n,
+                                  char* buf, int64_t cap);
+
+/* ------------------------------------------------------------ generators */
+/* Seeded synthetic graphs; each returns an owned gcl_graph. */
+gcl_status gcl_gen_erdos_renyi(int32_t n, double p, uint64_t seed, gcl_graph** out);
+gcl_status gcl_gen_erdos_renyi_mean_degree(int32_t n, double mean_degree, uint64_t seed,
+                                           gcl_graph** out);
+gcl_status gcl_gen_random_regular(int32_t n, int32_t d, uint64_t seed, gcl_graph** out);
+gcl_status gcl_gen_torus(int32_t a, int32_t b, gcl_graph** out);  /* C_a x C_b */
+gcl_status gcl_gen_cfi(int32_t base_n, int32_t twisted, uint64_t seed, gcl_graph** out);
+gcl_status gcl_gen_permute(const gcl_csr* g, uint64_t seed, gcl_graph** out);
+gcl_status gcl_gen_edge_swap(const gcl_csr* g, uint64_t seed, gcl_graph** out);
+
+/* ----------------------------------------------------------- device info */
+/* Number of visible GPUs (0 when none); never initialises a context when
+ * there is no driver. */
+int32_t gcl_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCL_H */
