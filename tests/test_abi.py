@@ -1,0 +1,58 @@
+"""The C-ABI library loads and exports every entry point include/gcl.h
+declares (no device calls)."""
+import os
+import re
+import subprocess
+
+import gcl
+from conftest import REPO
+
+
+def declared():
+    text = open(os.path.join(REPO, "include", "gcl.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gcl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("gcl_nauty_parallel", "gcl_nauty_sequential", "gcl_canon_label", "gcl_is_isomorphic",
+                 "gcl_batch_canon", "gcl_read_dimacs"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gcl.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", gcl.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    for name in declared():
+        assert name in exported, name
+        assert hasattr(lib, name)
+        assert name in gcl.exported_symbols(), name
+
+
+def test_status_strings_and_errors():
+    lib = gcl.lib()
+    assert lib.gcl_status_string(0) == b"ok"
+    bad = gcl.Graph(3, [0, 1, 1, 1], [1])  # asymmetric
+    try:
+        gcl.validate(bad)
+        raise AssertionError("expected GclError")
+    except gcl.GclError as e:
+        assert e.status == -1 and "symmetric" in str(e)
+
+
+def test_invalid_arguments_rejected_without_device():
+    g = gcl.torus(3, 3)
+    for kw in ({"max_aut": 0},):
+        try:
+            gcl.canon_label(g, engine="cpu", **kw)
+            raise AssertionError("expected GclError")
+        except gcl.GclError as e:
+            assert e.status == -1
+    empty = gcl.Graph(0, [0], [])
+    try:
+        gcl.canon_label(empty, engine="cpu")
+        raise AssertionError("expected GclError")
+    except gcl.GclError as e:
+        assert e.status == -1
