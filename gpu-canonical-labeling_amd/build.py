@@ -28,12 +28,12 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 CXXFLAGS = ["-O3", "-g", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
-            "-I" + INC, "-I" + CSRC]
+            "-I" + INC, "-I" + CSRC, "-pthread"]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-result", "-I" + INC, "-I" + CSRC]
+            "-Wno-unused-result", "-I" + INC, "-I" + CSRC, "-pthread"]
 
-HOST_SOURCES = ["gcl_api.cpp", "gcl_cpu.cpp", "gcl_graph.cpp"]
+HOST_SOURCES = ["gcl_api.cpp", "gcl_cpu.cpp", "gcl_graph.cpp", "gcl_workload.cpp"]
 DEVICE_SOURCES = ["gcl_device.hip"]
 
 
@@ -70,7 +70,7 @@ def build(with_reference=True):
         cmds.append(["make", "-s", "-B", "-C", ORACLE, "ref", "REF=" + REFERENCE])
     _run_all(cmds)
     lib = os.path.join(PKG, "libgcl.so")
-    _run_all([[HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib] + objs])
+    _run_all([[HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", lib] + objs])
     cli = os.path.join(PKG, "gpu_canon_label")
     _run_all([[CXX] + CXXFLAGS + [os.path.join(CSRC, "gpu_canon_label.cpp"), "-o", cli,
                                   "-L" + PKG, "-lgcl", "-Wl,-rpath,$ORIGIN"]])

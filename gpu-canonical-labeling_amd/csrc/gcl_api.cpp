@@ -250,6 +250,55 @@ gcl_status gcl_batch_canon(const gcl_batch* b, uint32_t max_aut, gcl_engine engi
     return GCL_OK;
 }
 
+gcl_status gcl_device_batch_create(const gcl_batch* b, uint32_t max_aut, gcl_device_batch** out) {
+    if (!out) return fail(GCL_E_INVALID, "null output handle");
+    gcl_status s = validate_batch(b);
+    if (s != GCL_OK) return s;
+    if ((s = check_max_aut(max_aut)) != GCL_OK) return s;
+    DeviceBatch* db = nullptr;
+    s = device_batch_create(*b, max_aut, kBatchRing, &db);
+    if (s != GCL_OK) return s;
+    *out = reinterpret_cast<gcl_device_batch*>(db);
+    return GCL_OK;
+}
+
+gcl_status gcl_device_batch_run(gcl_device_batch* db, int32_t use_graph, gcl_batch_stats* stats) {
+    if (!db) return fail(GCL_E_INVALID, "null handle");
+    auto t0 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    gcl_status s = device_batch_run(reinterpret_cast<DeviceBatch*>(db), use_graph != 0, &ms);
+    if (s != GCL_OK) return s;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return GCL_OK;
+}
+
+gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, uint64_t* out_nodes,
+                                    gcl_batch_stats* stats) {
+    if (!db) return fail(GCL_E_INVALID, "null handle");
+    DeviceBatch* d = reinterpret_cast<DeviceBatch*>(db);
+    gcl_batch_stats bs;
+    memset(&bs, 0, sizeof(bs));
+    std::vector<DeviceGraphResult> res;
+    if (out_nodes) res.resize((size_t)std::max(device_batch_size(d), 1));
+    gcl_status s = device_batch_results(d, out_label, out_nodes ? res.data() : nullptr, &bs);
+    if (out_nodes)
+        for (int32_t g = 0; g < device_batch_size(d); ++g) out_nodes[g] = res[(size_t)g].nodes;
+    if (stats) {
+        stats->nodes = bs.nodes;
+        stats->automorphisms = bs.automorphisms;
+        stats->splitter_steps = bs.splitter_steps;
+        stats->csr_bytes = bs.csr_bytes;
+        stats->workers = bs.workers;
+    }
+    return s;
+}
+
+void gcl_device_batch_destroy(gcl_device_batch* db) { device_batch_destroy(reinterpret_cast<DeviceBatch*>(db)); }
+
 gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut, int32_t* out_label, uint64_t* out_nodes,
                                  gcl_batch_stats* stats) {
     // One launch per graph, each with its own allocations, copies and

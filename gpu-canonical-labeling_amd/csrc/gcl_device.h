@@ -24,6 +24,14 @@ struct DeviceGraphResult {
 gcl_status gpu_canon_batch(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t* out_label,
                            DeviceGraphResult* results, gcl_batch_stats* st);
 
+// Batch resident on the device (opaque); see gcl_device_batch_* in gcl.h.
+struct DeviceBatch;
+gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out);
+gcl_status device_batch_run(DeviceBatch* db, bool use_graph, float* ms);
+gcl_status device_batch_results(DeviceBatch* db, int32_t* out_label, DeviceGraphResult* res, gcl_batch_stats* st);
+void device_batch_destroy(DeviceBatch* db);
+int32_t device_batch_size(const DeviceBatch* db);
+
 int32_t device_count();
 uint64_t step_budget();
 

@@ -1324,6 +1324,10 @@ __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
 // ------------------------------------------------------------------- host
 struct DevBuf {
     void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
@@ -1364,6 +1368,13 @@ const void* class_kernel(int cls) {
     }
 }
 
+// Zeroes the per-class work counters at the start of a run (a kernel rather
+// than a memset, so that the run can be captured in a graph).
+__global__ void reset_counters(unsigned int* ctr, int count) {
+    int i = (int)threadIdx.x;
+    if (i < count) ctr[i] = 0u;
+}
+
 }  // namespace
 
 // Per-graph bound on splitter steps plus search iterations; a graph that
@@ -1381,178 +1392,281 @@ int32_t device_count() {
     return c;
 }
 
+// A batch resident on device 0: CSR arrays, per-class worker scratch, output
+// buffers, streams and (optionally) a captured graph of one run.
+struct DeviceBatch {
+    struct Cls {
+        int cls = 0;
+        int32_t count = 0;
+        int64_t grid = 0;
+        size_t item_off = 0;
+        KParams P;
+        DevBuf scratch;
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    int32_t G = 0;
+    int64_t total_n = 0, total_nnz = 0;
+    uint32_t max_aut = 1, ring = 1;
+    DevBuf vtx, row, col, color, init, label, out, items, ctr;
+    std::vector<Cls> classes;
+    hipStream_t main = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, start = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t workers = 0;
+
+    ~DeviceBatch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& c : classes) {
+            if (c.done) (void)hipEventDestroy(c.done);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+        }
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (start) (void)hipEventDestroy(start);
+        if (main) (void)hipStreamDestroy(main);
+    }
+
+    gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap) {
+        if (max_aut_ == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
+        G = b.num_graphs;
+        if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
+        if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
+        GCL_HIP(hipSetDevice(0));
+        max_aut = max_aut_;
+        ring = std::max(1u, std::min(max_aut, ring_cap));
+        total_n = G > 0 ? b.vtx_off[G] : 0;
+        total_nnz = G > 0 ? b.row_ptr[total_n] : 0;
+
+        // size classes, larger graphs first within a class
+        std::vector<std::vector<int32_t>> items_h(kClasses + 1);
+        std::vector<uint32_t> cls_maxn(kClasses + 1, 0);
+        std::vector<uint64_t> cls_maxnnz(kClasses + 1, 0);
+        for (int32_t g = 0; g < G; ++g) {
+            const int64_t n = b.vtx_off[g + 1] - b.vtx_off[g];
+            if (n > kMaxVertices)
+                return fail(GCL_E_CAPACITY, "graph %d has n=%lld > %d", g, (long long)n, kMaxVertices);
+            int cls = kClasses;
+            for (int c = 0; c < kClasses; ++c)
+                if ((uint32_t)n <= kClassN[c]) {
+                    cls = c;
+                    break;
+                }
+            items_h[cls].push_back(g);
+            cls_maxn[cls] = std::max(cls_maxn[cls], (uint32_t)n);
+            const uint64_t nnz = (uint64_t)(b.row_ptr[b.vtx_off[g + 1]] - b.row_ptr[b.vtx_off[g]]);
+            cls_maxnnz[cls] = std::max(cls_maxnnz[cls], nnz);
+        }
+        for (auto& v : items_h)
+            std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
+                return b.vtx_off[x + 1] - b.vtx_off[x] > b.vtx_off[y + 1] - b.vtx_off[y];
+            });
+        const bool any_color = b.color != nullptr;
+        std::vector<int32_t> init_order;
+        if (any_color) {
+            init_order.resize((size_t)total_n);
+            for (int32_t g = 0; g < G; ++g) {
+                const int64_t v0 = b.vtx_off[g], v1 = b.vtx_off[g + 1];
+                for (int64_t v = v0; v < v1; ++v) init_order[(size_t)v] = (int32_t)(v - v0);
+                std::stable_sort(init_order.begin() + v0, init_order.begin() + v1,
+                                 [&](int32_t x, int32_t y) { return b.color[v0 + x] < b.color[v0 + y]; });
+            }
+        }
+        hipDeviceProp_t prop;
+        GCL_HIP(hipGetDeviceProperties(&prop, 0));
+        GCL_HIP(hipMalloc(&vtx.p, sizeof(int64_t) * (size_t)(G + 1)));
+        GCL_HIP(hipMalloc(&row.p, sizeof(int64_t) * (size_t)(total_n + 1)));
+        GCL_HIP(hipMalloc(&col.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_nnz, 1)));
+        GCL_HIP(hipMalloc(&label.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
+        GCL_HIP(hipMalloc(&out.p, sizeof(DeviceGraphResult) * (size_t)std::max(G, 1)));
+        GCL_HIP(hipMalloc(&items.p, sizeof(int32_t) * (size_t)std::max(G, 1)));
+        GCL_HIP(hipMalloc(&ctr.p, sizeof(unsigned int) * 64));
+        GCL_HIP(hipMemcpy(vtx.p, b.vtx_off, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice));
+        GCL_HIP(hipMemcpy(row.p, b.row_ptr, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice));
+        if (total_nnz > 0)
+            GCL_HIP(hipMemcpy(col.p, b.col, sizeof(int32_t) * (size_t)total_nnz, hipMemcpyHostToDevice));
+        if (any_color) {
+            GCL_HIP(hipMalloc(&color.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
+            GCL_HIP(hipMalloc(&init.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
+            GCL_HIP(hipMemcpy(color.p, b.color, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice));
+            GCL_HIP(hipMemcpy(init.p, init_order.data(), sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice));
+        }
+        std::vector<int32_t> flat;
+        for (auto& v : items_h) flat.insert(flat.end(), v.begin(), v.end());
+        if (!flat.empty())
+            GCL_HIP(hipMemcpy(items.p, flat.data(), sizeof(int32_t) * flat.size(), hipMemcpyHostToDevice));
+        GCL_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+        GCL_HIP(hipEventCreate(&ev0));
+        GCL_HIP(hipEventCreate(&ev1));
+        GCL_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+        size_t item_off = 0;
+        for (int cls = 0; cls <= kClasses; ++cls) {
+            const size_t cnt = items_h[cls].size();
+            if (cnt == 0) continue;
+            Cls C;
+            C.cls = cls;
+            C.count = (int32_t)cnt;
+            C.item_off = item_off;
+            const uint32_t maxn = cls < kClasses ? kClassN[cls] : cls_maxn[cls];
+            const uint64_t max_nnz = cls_maxnnz[cls];
+            // target-cell members over all levels of a path: n(n+1)/2 at worst,
+            // capped for the large classes (GCL_E_CAPACITY past the cap)
+            const uint64_t arena = std::min<uint64_t>((uint64_t)maxn * (maxn + 1) / 2, 64ull * maxn) + 64;
+            const bool hot_global = cls == kClasses;
+            const Layout L = make_layout(maxn, max_nnz, ring, arena, hot_global);
+            int per_cu = 0;
+            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls), kWave, 0));
+            if (per_cu <= 0) per_cu = 1;
+            const int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
+            size_t free_b = 0, total_b = 0;
+            GCL_HIP(hipMemGetInfo(&free_b, &total_b));
+            const int64_t mem_workers = (int64_t)((double)free_b * 0.5 / (double)L.total);
+            int64_t grid = std::min<int64_t>({(int64_t)cnt, resident, std::max<int64_t>(mem_workers, 1)});
+            C.grid = std::max<int64_t>(grid, 1);
+            GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
+            GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+            GCL_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
+            KParams& P = C.P;
+            P.num_items = (int32_t)cnt;
+            P.items = (const int32_t*)items.p + item_off;
+            P.vtx_off = (const int64_t*)vtx.p;
+            P.row_ptr = (const int64_t*)row.p;
+            P.col = (const int32_t*)col.p;
+            P.init_order = any_color ? (const int32_t*)init.p : nullptr;
+            P.color = any_color ? (const int32_t*)color.p : nullptr;
+            P.out_label = (int32_t*)label.p;
+            P.out = (DeviceGraphResult*)out.p;
+            P.next_item = (unsigned int*)ctr.p + cls;
+            P.scratch = (uint8_t*)C.scratch.p;
+            P.scratch_stride = L.total;
+            P.maxn = maxn;
+            P.max_nnz = max_nnz;
+            P.ring = ring;
+            P.max_aut = max_aut;
+            P.arena_cap = arena;
+            P.step_budget = step_budget();
+            workers += (int32_t)C.grid;
+            item_off += cnt;
+            classes.push_back(std::move(C));
+        }
+        return GCL_OK;
+    }
+
+    // reset counters, fork one stream per size class, join
+    hipError_t enqueue() {
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p, 64);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(start, main)) != hipSuccess) return e;
+        for (auto& c : classes) {
+            if ((e = hipStreamWaitEvent(c.stream, start, 0)) != hipSuccess) return e;
+            if ((e = launch_class(c.cls, c.P, (int)c.grid, c.stream)) != hipSuccess) return e;
+            if ((e = hipEventRecord(c.done, c.stream)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(main, c.done, 0)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+
+    gcl_status run(bool use_graph, float* ms) {
+        GCL_HIP(hipSetDevice(0));
+        if (use_graph && !exec) {
+            GCL_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeGlobal));
+            hipError_t e = enqueue();
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(main, &gr);
+            if (e != hipSuccess) return fail(GCL_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(GCL_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
+            graph = gr;
+            GCL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        GCL_HIP(hipEventRecord(ev0, main));
+        if (use_graph) {
+            GCL_HIP(hipGraphLaunch(exec, main));
+        } else {
+            hipError_t e = enqueue();
+            if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+        }
+        GCL_HIP(hipEventRecord(ev1, main));
+        hipError_t se = hipStreamSynchronize(main);
+        if (se != hipSuccess) return fail(GCL_E_DEVICE, "search kernel failed: %s", hipGetErrorString(se));
+        float t = 0.f;
+        GCL_HIP(hipEventElapsedTime(&t, ev0, ev1));
+        if (ms) *ms = t;
+        return GCL_OK;
+    }
+
+    gcl_status results(int32_t* out_label, DeviceGraphResult* res_out, gcl_batch_stats* st) {
+        GCL_HIP(hipSetDevice(0));
+        std::vector<DeviceGraphResult> res((size_t)std::max(G, 1));
+        if (G > 0)
+            GCL_HIP(hipMemcpy(res.data(), out.p, sizeof(DeviceGraphResult) * (size_t)G, hipMemcpyDeviceToHost));
+        if (out_label && total_n > 0)
+            GCL_HIP(hipMemcpy(out_label, label.p, sizeof(int32_t) * (size_t)total_n, hipMemcpyDeviceToHost));
+        uint64_t nodes = 0, auts = 0, steps = 0, bytes = 0;
+        gcl_status worst = GCL_OK;
+        for (int32_t g = 0; g < G; ++g) {
+            nodes += res[(size_t)g].nodes;
+            auts += res[(size_t)g].automorphisms;
+            steps += res[(size_t)g].splitter_steps;
+            bytes += res[(size_t)g].csr_bytes;
+            if (res[(size_t)g].status != 0 && worst == GCL_OK) {
+                worst = (gcl_status)res[(size_t)g].status;
+                set_error("graph %d: device search failed (%s)", g, gcl_status_string(worst));
+            }
+            if (res_out) res_out[g] = res[(size_t)g];
+        }
+        if (st) {
+            st->nodes = nodes;
+            st->automorphisms = auts;
+            st->splitter_steps = steps;
+            st->csr_bytes = bytes;
+            st->workers = workers;
+        }
+        return worst;
+    }
+};
+
+gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out) {
+    DeviceBatch* db = new (std::nothrow) DeviceBatch();
+    if (!db) return fail(GCL_E_NOMEM, "out of host memory");
+    gcl_status s = db->create(b, max_aut, ring_cap);
+    if (s != GCL_OK) {
+        delete db;
+        return s;
+    }
+    *out = db;
+    return GCL_OK;
+}
+
+gcl_status device_batch_run(DeviceBatch* db, bool use_graph, float* ms) { return db->run(use_graph, ms); }
+
+gcl_status device_batch_results(DeviceBatch* db, int32_t* out_label, DeviceGraphResult* res, gcl_batch_stats* st) {
+    return db->results(out_label, res, st);
+}
+
+void device_batch_destroy(DeviceBatch* db) { delete db; }
+
+int32_t device_batch_size(const DeviceBatch* db) { return db->G; }
+
 gcl_status gpu_canon_batch(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t* out_label,
                            DeviceGraphResult* results, gcl_batch_stats* st) {
     auto t0 = std::chrono::steady_clock::now();
-    if (max_aut == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
-    const int32_t G = b.num_graphs;
-    if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
-    if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
-    GCL_HIP(hipSetDevice(0));
-    const int64_t total_n = G > 0 ? b.vtx_off[G] : 0;
-    const int64_t total_nnz = G > 0 ? b.row_ptr[total_n] : 0;
-    const uint32_t ring = std::max(1u, std::min(max_aut, ring_cap));
-
-    // size classes and per-class item lists, larger graphs first
-    std::vector<std::vector<int32_t>> items(kClasses + 1);
-    std::vector<uint32_t> cls_maxn(kClasses + 1, 0);
-    std::vector<uint64_t> cls_maxnnz(kClasses + 1, 0);
-    bool any_color = b.color != nullptr;
-    for (int32_t g = 0; g < G; ++g) {
-        const int64_t n = b.vtx_off[g + 1] - b.vtx_off[g];
-        if (n > kMaxVertices) return fail(GCL_E_CAPACITY, "graph %d has n=%lld > %d", g, (long long)n, kMaxVertices);
-        int cls = kClasses;
-        for (int c = 0; c < kClasses; ++c)
-            if ((uint32_t)n <= kClassN[c]) {
-                cls = c;
-                break;
-            }
-        items[cls].push_back(g);
-        cls_maxn[cls] = std::max(cls_maxn[cls], (uint32_t)n);
-        const uint64_t nnz = (uint64_t)(b.row_ptr[b.vtx_off[g + 1]] - b.row_ptr[b.vtx_off[g]]);
-        cls_maxnnz[cls] = std::max(cls_maxnnz[cls], nnz);
-    }
-    for (auto& v : items)
-        std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
-            return b.vtx_off[x + 1] - b.vtx_off[x] > b.vtx_off[y + 1] - b.vtx_off[y];
-        });
-
-    // initial order for coloured graphs: stable by colour within each graph
-    std::vector<int32_t> init_order;
-    if (any_color) {
-        init_order.resize((size_t)total_n);
-        for (int32_t g = 0; g < G; ++g) {
-            const int64_t v0 = b.vtx_off[g], v1 = b.vtx_off[g + 1];
-            for (int64_t v = v0; v < v1; ++v) init_order[(size_t)v] = (int32_t)(v - v0);
-            std::stable_sort(init_order.begin() + v0, init_order.begin() + v1,
-                             [&](int32_t x, int32_t y) { return b.color[v0 + x] < b.color[v0 + y]; });
-        }
-    }
-
-    hipDeviceProp_t prop;
-    GCL_HIP(hipGetDeviceProperties(&prop, 0));
-
-    DevBuf d_vtx, d_row, d_col, d_color, d_init, d_label, d_out, d_items, d_ctr;
-    GCL_HIP(hipMalloc(&d_vtx.p, sizeof(int64_t) * (size_t)(G + 1)));
-    GCL_HIP(hipMalloc(&d_row.p, sizeof(int64_t) * (size_t)(total_n + 1)));
-    GCL_HIP(hipMalloc(&d_col.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_nnz, 1)));
-    GCL_HIP(hipMalloc(&d_label.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
-    GCL_HIP(hipMalloc(&d_out.p, sizeof(DeviceGraphResult) * (size_t)std::max(G, 1)));
-    GCL_HIP(hipMalloc(&d_items.p, sizeof(int32_t) * (size_t)std::max(G, 1)));
-    GCL_HIP(hipMalloc(&d_ctr.p, sizeof(unsigned int) * (kClasses + 1)));
-    GCL_HIP(hipMemcpy(d_vtx.p, b.vtx_off, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice));
-    GCL_HIP(hipMemcpy(d_row.p, b.row_ptr, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice));
-    if (total_nnz > 0)
-        GCL_HIP(hipMemcpy(d_col.p, b.col, sizeof(int32_t) * (size_t)total_nnz, hipMemcpyHostToDevice));
-    if (any_color) {
-        GCL_HIP(hipMalloc(&d_color.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
-        GCL_HIP(hipMalloc(&d_init.p, sizeof(int32_t) * (size_t)std::max<int64_t>(total_n, 1)));
-        GCL_HIP(hipMemcpy(d_color.p, b.color, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice));
-        GCL_HIP(hipMemcpy(d_init.p, init_order.data(), sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice));
-    }
-    GCL_HIP(hipMemset(d_ctr.p, 0, sizeof(unsigned int) * (kClasses + 1)));
-    {
-        std::vector<int32_t> flat;
-        for (auto& v : items) flat.insert(flat.end(), v.begin(), v.end());
-        if (!flat.empty())
-            GCL_HIP(hipMemcpy(d_items.p, flat.data(), sizeof(int32_t) * flat.size(), hipMemcpyHostToDevice));
-    }
-
-    // per-class worker scratch and launch
-    std::vector<DevBuf> scratch(kClasses + 1);
-    std::vector<hipStream_t> streams(kClasses + 1, nullptr);
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    GCL_HIP(hipEventCreate(&ev0));
-    GCL_HIP(hipEventCreate(&ev1));
-    int32_t total_workers = 0;
-    size_t item_off = 0;
-    gcl_status rc = GCL_OK;
-    GCL_HIP(hipEventRecord(ev0, 0));
-    for (int cls = 0; cls <= kClasses && rc == GCL_OK; ++cls) {
-        const size_t cnt = items[cls].size();
-        if (cnt == 0) continue;
-        const uint32_t maxn = cls < kClasses ? kClassN[cls] : cls_maxn[cls];
-        const uint64_t max_nnz = cls_maxnnz[cls];
-        // target-cell members over all levels of a path: n(n+1)/2 at worst,
-        // capped for the large classes (GCL_E_CAPACITY past the cap)
-        const uint64_t arena = std::min<uint64_t>((uint64_t)maxn * (maxn + 1) / 2, 64ull * maxn) + 64;
-        const bool hot_global = cls == kClasses;
-        const Layout L = make_layout(maxn, max_nnz, ring, arena, hot_global);
-        int per_cu = 0;
-        GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls), kWave, 0));
-        if (per_cu <= 0) per_cu = 1;
-        int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
-        // bound scratch to a fraction of free memory
-        size_t free_b = 0, total_b = 0;
-        GCL_HIP(hipMemGetInfo(&free_b, &total_b));
-        int64_t mem_workers = (int64_t)((double)free_b * 0.5 / (double)L.total);
-        int64_t grid = std::min<int64_t>({(int64_t)cnt, resident, std::max<int64_t>(mem_workers, 1)});
-        if (grid < 1) grid = 1;
-        GCL_HIP(hipMalloc(&scratch[cls].p, (size_t)(L.total * (uint64_t)grid)));
-        GCL_HIP(hipStreamCreateWithFlags(&streams[cls], hipStreamNonBlocking));
-        KParams P;
-        P.num_items = (int32_t)cnt;
-        P.items = (const int32_t*)d_items.p + item_off;
-        P.vtx_off = (const int64_t*)d_vtx.p;
-        P.row_ptr = (const int64_t*)d_row.p;
-        P.col = (const int32_t*)d_col.p;
-        P.init_order = any_color ? (const int32_t*)d_init.p : nullptr;
-        P.color = any_color ? (const int32_t*)d_color.p : nullptr;
-        P.out_label = (int32_t*)d_label.p;
-        P.out = (DeviceGraphResult*)d_out.p;
-        P.next_item = (unsigned int*)d_ctr.p + cls;
-        P.scratch = (uint8_t*)scratch[cls].p;
-        P.scratch_stride = L.total;
-        P.maxn = maxn;
-        P.max_nnz = max_nnz;
-        P.ring = ring;
-        P.max_aut = max_aut;
-        P.arena_cap = arena;
-        P.step_budget = step_budget();
-        hipError_t e = launch_class(cls, P, (int)grid, streams[cls]);
-        if (e != hipSuccess) rc = fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
-        total_workers += (int32_t)grid;
-        item_off += cnt;
-    }
-    hipError_t se = hipDeviceSynchronize();
-    GCL_HIP(hipEventRecord(ev1, 0));
-    GCL_HIP(hipEventSynchronize(ev1));
+    DeviceBatch* db = nullptr;
+    gcl_status s = device_batch_create(b, max_aut, ring_cap, &db);
+    if (s != GCL_OK) return s;
     float ms = 0.f;
-    GCL_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    for (auto s : streams)
-        if (s) (void)hipStreamDestroy(s);
-    if (rc != GCL_OK) return rc;
-    if (se != hipSuccess) return fail(GCL_E_DEVICE, "search kernel failed: %s", hipGetErrorString(se));
-
-    std::vector<DeviceGraphResult> res((size_t)std::max(G, 1));
-    GCL_HIP(hipMemcpy(res.data(), d_out.p, sizeof(DeviceGraphResult) * (size_t)G, hipMemcpyDeviceToHost));
-    if (out_label && total_n > 0)
-        GCL_HIP(hipMemcpy(out_label, d_label.p, sizeof(int32_t) * (size_t)total_n, hipMemcpyDeviceToHost));
-    uint64_t nodes = 0, auts = 0, steps = 0, bytes = 0;
-    gcl_status worst = GCL_OK;
-    for (int32_t g = 0; g < G; ++g) {
-        nodes += res[(size_t)g].nodes;
-        auts += res[(size_t)g].automorphisms;
-        steps += res[(size_t)g].splitter_steps;
-        bytes += res[(size_t)g].csr_bytes;
-        if (res[(size_t)g].status != 0 && worst == GCL_OK) {
-            worst = (gcl_status)res[(size_t)g].status;
-            set_error("graph %d: device search failed (%s)", g, gcl_status_string(worst));
-        }
-        if (results) results[g] = res[(size_t)g];
-    }
-    if (st) {
-        st->nodes = nodes;
-        st->automorphisms = auts;
-        st->splitter_steps = steps;
-        st->csr_bytes = bytes;
-        st->kernel_ms = ms;
-        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        st->workers = total_workers;
-    }
-    return worst;
+    s = db->run(false, &ms);
+    gcl_batch_stats bs;
+    memset(&bs, 0, sizeof(bs));
+    if (s == GCL_OK) s = db->results(out_label, results, &bs);
+    delete db;
+    if (s != GCL_OK) return s;
+    bs.kernel_ms = ms;
+    bs.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = bs;
+    return GCL_OK;
 }
 
 }  // namespace gcl

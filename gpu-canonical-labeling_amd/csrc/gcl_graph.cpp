@@ -149,6 +149,43 @@ void keys_to_dense(const uint32_t* keys, int64_t nnz, int32_t n, uint8_t* dense)
     for (int64_t k = 0; k < nnz; ++k) dense[keys[k]] = 1;
 }
 
+gcl_status random_regular_pairs(int32_t n, int32_t d, uint64_t seed,
+                                std::vector<std::pair<int32_t, int32_t>>& pairs) {
+    // Configuration model: a uniform perfect matching of the n*d stubs,
+    // drawn pair by pair and restarted at the first loop or repeated edge,
+    // which gives the uniform distribution over simple d-regular graphs.
+    if (n < 1 || n > kMaxVertices || d < 0 || d >= n || ((int64_t)n * d) % 2 != 0)
+        return fail(GCL_E_INVALID, "no simple %d-regular graph on %d vertices", d, n);
+    Rng rng(seed);
+    const int64_t stubs = (int64_t)n * d;
+    std::vector<int32_t> pts((size_t)stubs);
+    std::vector<int32_t> nbr((size_t)stubs);
+    std::vector<int32_t> fill((size_t)n);
+    for (int attempt = 0; attempt < 10000000; ++attempt) {
+        for (int64_t i = 0; i < stubs; ++i) pts[(size_t)i] = (int32_t)(i / d);
+        std::fill(fill.begin(), fill.end(), 0);
+        pairs.clear();
+        bool ok = true;
+        for (int64_t i = 0; i < stubs && ok; i += 2) {
+            const int64_t j = i + 1 + (int64_t)rng.below((uint64_t)(stubs - i - 1));
+            std::swap(pts[(size_t)i + 1], pts[(size_t)j]);
+            const int32_t a = pts[(size_t)i], b = pts[(size_t)i + 1];
+            if (a == b) {
+                ok = false;
+                break;
+            }
+            for (int32_t k = 0; k < fill[(size_t)a]; ++k)
+                if (nbr[(size_t)a * d + k] == b) ok = false;
+            if (!ok) break;
+            nbr[(size_t)a * d + fill[(size_t)a]++] = b;
+            nbr[(size_t)b * d + fill[(size_t)b]++] = a;
+            pairs.push_back({std::min(a, b), std::max(a, b)});
+        }
+        if (ok) return GCL_OK;
+    }
+    return fail(GCL_E_INTERNAL, "configuration model did not produce a simple graph");
+}
+
 }  // namespace gcl
 
 using namespace gcl;
@@ -358,30 +395,10 @@ gcl_status gcl_gen_erdos_renyi_mean_degree(int32_t n, double mean_degree, uint64
 }
 
 gcl_status gcl_gen_random_regular(int32_t n, int32_t d, uint64_t seed, gcl_graph** out) {
-    if (n < 1 || n > kMaxVertices || d < 0 || d >= n || ((int64_t)n * d) % 2 != 0)
-        return fail(GCL_E_INVALID, "no simple %d-regular graph on %d vertices", d, n);
-    Rng rng(seed);
-    const int64_t stubs = (int64_t)n * d;
-    std::vector<int32_t> pts((size_t)stubs);
     std::vector<std::pair<int32_t, int32_t>> pairs;
-    for (int attempt = 0; attempt < 1000000; ++attempt) {
-        // configuration model: uniform pairing of stubs, rejected unless simple
-        for (int64_t i = 0; i < stubs; ++i) pts[(size_t)i] = (int32_t)(i / d);
-        for (int64_t i = stubs - 1; i > 0; --i) std::swap(pts[(size_t)i], pts[(size_t)rng.below((uint64_t)i + 1)]);
-        pairs.clear();
-        bool ok = true;
-        for (int64_t i = 0; i < stubs && ok; i += 2) {
-            int32_t a = pts[(size_t)i], b = pts[(size_t)i + 1];
-            if (a == b) ok = false;
-            else pairs.push_back({std::min(a, b), std::max(a, b)});
-        }
-        if (!ok) continue;
-        std::vector<std::pair<int32_t, int32_t>> s = pairs;
-        std::sort(s.begin(), s.end());
-        if (std::adjacent_find(s.begin(), s.end()) != s.end()) continue;
-        return graph_to_owned(graph_from_pairs(n, pairs), out);
-    }
-    return fail(GCL_E_INTERNAL, "configuration model did not produce a simple graph");
+    gcl_status st = random_regular_pairs(n, d, seed, pairs);
+    if (st != GCL_OK) return st;
+    return graph_to_owned(graph_from_pairs(n, pairs), out);
 }
 
 gcl_status gcl_gen_torus(int32_t a, int32_t b, gcl_graph** out) {

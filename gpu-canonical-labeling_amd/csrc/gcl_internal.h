@@ -50,6 +50,10 @@ void canonical_keys(const gcl_csr& g, const int32_t* label, uint32_t* keys);
 // Dense n*n matrix from sorted keys.
 void keys_to_dense(const uint32_t* keys, int64_t nnz, int32_t n, uint8_t* dense);
 
+// Edges of a uniform random simple d-regular graph (configuration model).
+gcl_status random_regular_pairs(int32_t n, int32_t d, uint64_t seed,
+                                std::vector<std::pair<int32_t, int32_t>>& pairs);
+
 // Largest n the library accepts (keys i*n+j must fit in 32 bits).
 constexpr int32_t kMaxVertices = 65535;
 

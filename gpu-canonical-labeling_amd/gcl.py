@@ -86,6 +86,12 @@ _SIGS = {
     "gcl_gen_permute": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "gcl_gen_edge_swap": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "gcl_device_count": (C.c_int32, []),
+    "gcl_device_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "gcl_device_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gcl_device_batch_destroy": (None, [C.c_void_p]),
+    "gcl_gen_mix_batch": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "gcl_owned_batch_free": (None, [C.c_void_p]),
 }
 
 
@@ -297,8 +303,36 @@ def format_adjacency(keys: np.ndarray, n: int) -> str:
     return buf.raw[:ln].decode()
 
 
+class _OwnedBatch(C.Structure):
+    _fields_ = [("batch", _Batch), ("vtx_buf", C.c_void_p), ("row_buf", C.c_void_p), ("col_buf", C.c_void_p),
+                ("kind", C.c_void_p)]
+
+
 class Batch:
     """A flat batch of graphs (the layout of gcl_batch in include/gcl.h)."""
+
+    @staticmethod
+    def from_arrays(vtx_off, row_ptr, col, color=None, kind=None):
+        b = Batch.__new__(Batch)
+        b.graphs = None
+        b.vtx_off = np.ascontiguousarray(vtx_off, np.int64)
+        b.row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        b.col = np.ascontiguousarray(col, np.int32)
+        b.color = None if color is None else np.ascontiguousarray(color, np.int32)
+        b.kind = kind
+        return b
+
+    @property
+    def num_graphs(self):
+        return len(self.vtx_off) - 1
+
+    def graph(self, i) -> Graph:
+        v0, v1 = int(self.vtx_off[i]), int(self.vtx_off[i + 1])
+        e0 = int(self.row_ptr[v0])
+        rp = (self.row_ptr[v0:v1 + 1] - e0).astype(np.int32)
+        col = self.col[e0:int(self.row_ptr[v1])].copy()
+        color = None if self.color is None else self.color[v0:v1].copy()
+        return Graph(v1 - v0, rp, col, color)
 
     def __init__(self, graphs: Sequence[Graph]):
         self.graphs = list(graphs)
@@ -320,7 +354,7 @@ class Batch:
         self.color = np.ascontiguousarray(np.concatenate(colors)) if any_color else None
 
     def _c(self):
-        return _Batch(len(self.graphs), _ptr(self.vtx_off), _ptr(self.row_ptr),
+        return _Batch(self.num_graphs, _ptr(self.vtx_off), _ptr(self.row_ptr),
                       _ptr(self.col) if self.col.size else None, _ptr(self.color))
 
     def labels_of(self, labels, i):
@@ -340,17 +374,69 @@ class BatchResult:
     workers: int
 
 
+def mix_batch(count, seed, n_min=16, n_max=1024, threads=0) -> Batch:
+    """The batched benchmark mix (gcl_gen_mix_batch)."""
+    ob = C.POINTER(_OwnedBatch)()
+    _check(lib().gcl_gen_mix_batch(count, seed, n_min, n_max, threads, C.byref(ob)))
+    o = ob.contents
+    G = o.batch.num_graphs
+    vtx = np.ctypeslib.as_array(C.cast(o.vtx_buf, C.POINTER(C.c_int64)), (G + 1,)).copy()
+    tn = int(vtx[-1])
+    rows = np.ctypeslib.as_array(C.cast(o.row_buf, C.POINTER(C.c_int64)), (tn + 1,)).copy()
+    nnz = int(rows[-1])
+    col = (np.ctypeslib.as_array(C.cast(o.col_buf, C.POINTER(C.c_int32)), (nnz,)).copy()
+           if nnz else np.zeros(0, np.int32))
+    kind = np.ctypeslib.as_array(C.cast(o.kind, C.POINTER(C.c_int32)), (max(G, 1),)).copy()[:G]
+    lib().gcl_owned_batch_free(ob)
+    return Batch.from_arrays(vtx, rows, col, None, kind)
+
+
+class DeviceBatch:
+    """A batch resident in device memory (gcl_device_batch_*)."""
+
+    def __init__(self, batch: Batch, max_aut=10000):
+        self.batch = batch
+        self._c = batch._c()
+        self.h = C.c_void_p()
+        _check(lib().gcl_device_batch_create(C.byref(self._c), max_aut, C.byref(self.h)))
+
+    def run(self, use_graph=False):
+        st = _BatchStats()
+        _check(lib().gcl_device_batch_run(self.h, 1 if use_graph else 0, C.byref(st)))
+        return st.kernel_ms, st.total_ms
+
+    def results(self):
+        b = self.batch
+        labels = np.zeros(max(int(b.vtx_off[-1]), 1), np.int32)
+        nodes = np.zeros(max(b.num_graphs, 1), np.uint64)
+        st = _BatchStats()
+        _check(lib().gcl_device_batch_results(self.h, _ptr(labels), _ptr(nodes), C.byref(st)))
+        return BatchResult(labels[:int(b.vtx_off[-1])], nodes[:b.num_graphs], st.nodes, st.automorphisms,
+                           st.splitter_steps, st.csr_bytes, 0.0, 0.0, st.workers)
+
+    def close(self):
+        if self.h:
+            lib().gcl_device_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def batch_canon(batch: Batch, max_aut=10000, engine="gpu", plain=False) -> BatchResult:
     c = batch._c()
     labels = np.zeros(max(int(batch.vtx_off[-1]), 1), np.int32)
-    nodes = np.zeros(max(len(batch.graphs), 1), np.uint64)
+    nodes = np.zeros(max(batch.num_graphs, 1), np.uint64)
     st = _BatchStats()
     if plain:
         _check(lib().gcl_batch_canon_plain(C.byref(c), max_aut, _ptr(labels), _ptr(nodes), C.byref(st)))
     else:
         _check(lib().gcl_batch_canon(C.byref(c), max_aut, ENGINES[engine], _ptr(labels), _ptr(nodes),
                                      C.byref(st)))
-    return BatchResult(labels[:int(batch.vtx_off[-1])], nodes[:len(batch.graphs)], st.nodes, st.automorphisms,
+    return BatchResult(labels[:int(batch.vtx_off[-1])], nodes[:batch.num_graphs], st.nodes, st.automorphisms,
                        st.splitter_steps, st.csr_bytes, st.kernel_ms, st.total_ms, st.workers)
 
 

@@ -153,6 +153,24 @@ gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut,
                                  int32_t* out_label, uint64_t* out_nodes,
                                  gcl_batch_stats* stats);
 
+/*
+ * Batch resident on the device: the CSR arrays, per-worker scratch and
+ * outputs stay in HBM between runs, so a run measures the search alone.
+ *   create   validates, uploads and sizes the launch (max_aut as above)
+ *   run      one persistent launch per size class, concurrent on their own
+ *            streams; use_graph != 0 captures the run as a hipGraph on first
+ *            use and replays it afterwards.  stats gets kernel_ms and workers
+ *   results  copies labels (vtx_off[G] ints) and per-graph node counts back
+ *            and fills the totals of stats; either pointer may be NULL
+ * Not thread-safe per handle.
+ */
+typedef struct gcl_device_batch gcl_device_batch;
+gcl_status gcl_device_batch_create(const gcl_batch* b, uint32_t max_aut, gcl_device_batch** out);
+gcl_status gcl_device_batch_run(gcl_device_batch* db, int32_t use_graph, gcl_batch_stats* stats);
+gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, uint64_t* out_nodes,
+                                    gcl_batch_stats* stats);
+void gcl_device_batch_destroy(gcl_device_batch* db);
+
 /* Canonical keys of one graph from a labeling: out_keys[nnz] = sorted
  * i*n+j over directed edges of the relabeled graph. */
 gcl_status gcl_canonical_keys(const gcl_csr* g, const int32_t* label, uint32_t* out_keys);
@@ -190,6 +208,26 @@ gcl_status gcl_gen_torus(int32_t a, int32_t b, gcl_graph** out);  /* C_a x C_b *
 gcl_status gcl_gen_cfi(int32_t base_n, int32_t twisted, uint64_t seed, gcl_graph** out);
 gcl_status gcl_gen_permute(const gcl_csr* g, uint64_t seed, gcl_graph** out);
 gcl_status gcl_gen_edge_swap(const gcl_csr* g, uint64_t seed, gcl_graph** out);
+
+/* Owned flat batch returned by the workload generator. */
+typedef struct gcl_owned_batch {
+    gcl_batch batch;
+    int64_t* vtx_buf;
+    int64_t* row_buf;
+    int32_t* col_buf;
+    int32_t* kind;        /* per graph: 0 G(n,p), 1 random regular, 2 C_a x C_b */
+} gcl_owned_batch;
+
+/*
+ * The batched benchmark mix: `count` graphs; n log-uniform in [n_min, n_max];
+ * 50% G(n,p) with mean degree uniform in [3,12], 30% random d-regular with d
+ * in {3,4,5} (n raised by one when n*d is odd), 20% C_a x C_b with a, b
+ * uniform in [4,32]; every graph under a seeded random relabeling.  Graph i
+ * depends only on (seed, i); `threads` <= 0 uses the hardware concurrency.
+ */
+gcl_status gcl_gen_mix_batch(int32_t count, uint64_t seed, int32_t n_min, int32_t n_max,
+                             int32_t threads, gcl_owned_batch** out);
+void gcl_owned_batch_free(gcl_owned_batch* b);
 
 /* ----------------------------------------------------------- device info */
 /* Number of visible GPUs (0 when none); never initialises a context when
