@@ -1,0 +1,169 @@
+"""Benchmark: canonical labelings/s of the batched mix on one MI355X.
+
+Workload (one "step"): canonical labeling of a fixed batch of 65,536 graphs,
+n log-uniform in [16, 1024]; 50% G(n,p) with mean degree U[3,12], 30% random
+d-regular (d in {3,4,5}), 20% C_a x C_b (a, b in U[4,32]); each under a seeded
+random relabeling.  The batch is uploaded once (inputs resident in HBM); a
+step is one run of the persistent search launch (one kernel per size class,
+replayed from a captured hipGraph) and ends with a stream synchronize.
+
+Printed as one JSON line with the throughput, the plain one-graph-per-launch
+rate on a sample of the same batch, the HBM roofline figure of the search
+launch and the CPU baseline (the reference's sequential binary, or the oracle
+where that binary is absent, on a bounded sample, one core).
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gpu-canonical-labeling_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+import numpy as np  # noqa: E402
+
+import gcl  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "canonical labelings/sec + p99 latency, batched/served vs plain GPU, bit-exact"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE), committed under profiles/."""
+    path = os.path.join(REPO, "profiles", "search_batch_mix_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+RE_TIME = re.compile(r"Algorithm ran in: (\d+)min, (\d+)s, (\d+)ms, (\d+)\S+s, (\d+)ns")
+
+
+def cpu_baseline(batch, budget_s, stride):
+    """Times the reference's sequential binary (or the oracle) on graphs
+    0, stride, 2*stride, ... of the batch until budget_s of CPU time is spent."""
+    import oracle
+    ref = oracle.REF_BIN
+    sample = list(range(0, batch.num_graphs, stride))
+    done, spent = 0, 0.0
+    kind = "reference" if os.path.exists(ref) else "port"
+    with tempfile.TemporaryDirectory() as tmp:
+        for i in sample:
+            g = batch.graph(i)
+            if kind == "reference":
+                f = os.path.join(tmp, "g.dimacs")
+                gcl.write_dimacs(f, g)
+                p = subprocess.run([ref, f, "--sequential", "--max-aut=10000"], cwd=tmp, capture_output=True,
+                                   text=True, timeout=max(60, int(budget_s * 4)))
+                m = RE_TIME.search(p.stdout)
+                if p.returncode != 0 or not m:
+                    raise RuntimeError("reference run failed: " + p.stderr[-300:])
+                mi, s, ms, us, ns = (int(x) for x in m.groups())
+                spent += mi * 60 + s + ms * 1e-3 + us * 1e-6 + ns * 1e-9
+            else:
+                t0 = time.perf_counter()
+                oracle.canon_graph(g, want_dense=False)
+                spent += time.perf_counter() - t0
+            done += 1
+            if spent >= budget_s:
+                break
+    return dict(value=done / spent, unit="labelings/s", cores=1, kind=kind,
+                sample=f"graphs 0,{stride},{2 * stride},... of the same batch: {done} graphs, {spent:.1f} s "
+                       f"of single-thread algorithm time ({'reference sequential binary' if kind == 'reference' else 'oracle'})")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--graphs", type=int, default=65536)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-graph", action="store_true", help="launch without hipGraph replay")
+    ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--skip-baselines", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "graphs are independent, so a batch shards across GPUs without exchange.", file=sys.stderr)
+        sys.exit(2)
+    if gcl.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+
+    t0 = time.time()
+    batch = gcl.mix_batch(args.graphs, args.seed)
+    G = batch.num_graphs
+    log(f"generated {G} graphs, n total {int(batch.vtx_off[-1])}, nnz {int(batch.row_ptr[-1])} "
+        f"in {time.time() - t0:.1f}s")
+    db = gcl.DeviceBatch(batch, max_aut=10000)
+    use_graph = not args.no_graph
+    for w in range(args.warmup):
+        k, t = db.run(use_graph=use_graph)
+        log(f"warmup {w}: kernel {k:.1f} ms, wall {t:.1f} ms")
+    walls, kernels = [], []
+    for s in range(args.steps):
+        k, t = db.run(use_graph=use_graph)
+        walls.append(t)
+        kernels.append(k)
+        log(f"step {s}: kernel {k:.1f} ms, wall {t:.1f} ms")
+    res = db.results()
+    total_s = sum(walls) / 1e3
+    value = args.steps * G / total_s
+    ms_per_step = sum(walls) / args.steps
+    kernel_ms = sum(kernels) / args.steps
+    achieved = res.csr_bytes / (kernel_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "labelings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": f"batched mix: {G} graphs, n log-uniform [16,1024], 50% G(n,p) deg U[3,12], "
+                               "30% random d-regular d in {3,4,5}, 20% C_a x C_b a,b in U[4,32], seeded relabeling",
+                   "graphs": G, "seed": args.seed, "hipgraph": use_graph, "workers": res.workers},
+        "search_nodes_per_s": round(res.total_nodes / (kernel_ms / 1e3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic_from_profile()},
+    }
+    if not args.skip_baselines:
+        # plain path: one launch per graph (allocation, copies, launch, sync)
+        ps = min(args.plain_sample, G)
+        idx = np.linspace(0, G - 1, ps).astype(np.int64)
+        sub = gcl.Batch([batch.graph(int(i)) for i in idx])
+        t1 = time.perf_counter()
+        plain = gcl.batch_canon(sub, plain=True)
+        t_plain = time.perf_counter() - t1
+        for j, i in enumerate(idx):
+            a = sub.labels_of(plain.labels, j)
+            b = res.labels[batch.vtx_off[i]:batch.vtx_off[i + 1]]
+            if not np.array_equal(a, b):
+                raise SystemExit(f"bench.py: plain and batched labelings differ on graph {i}")
+        plain_rate = ps / t_plain
+        out["plain"] = {"value": round(plain_rate, 2), "unit": "labelings/s",
+                        "sample": f"{ps} graphs evenly spaced over the batch, one launch each"}
+        out["batched_vs_plain"] = round(value / plain_rate, 2)
+        out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 512)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
