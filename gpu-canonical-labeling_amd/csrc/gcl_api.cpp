@@ -297,6 +297,22 @@ gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, ui
     return s;
 }
 
+gcl_status gcl_device_batch_graph_stats(gcl_device_batch* db, uint64_t* nodes, uint64_t* automorphisms,
+                                        uint64_t* splitter_steps, uint64_t* device_ns) {
+    if (!db) return fail(GCL_E_INVALID, "null handle");
+    DeviceBatch* d = reinterpret_cast<DeviceBatch*>(db);
+    const int32_t G = device_batch_size(d);
+    std::vector<DeviceGraphResult> res((size_t)std::max(G, 1));
+    gcl_status s = device_batch_results(d, nullptr, res.data(), nullptr);
+    for (int32_t g = 0; g < G; ++g) {
+        if (nodes) nodes[g] = res[(size_t)g].nodes;
+        if (automorphisms) automorphisms[g] = res[(size_t)g].automorphisms;
+        if (splitter_steps) splitter_steps[g] = res[(size_t)g].splitter_steps;
+        if (device_ns) device_ns[g] = res[(size_t)g].device_ns;
+    }
+    return s;
+}
+
 void gcl_device_batch_destroy(gcl_device_batch* db) { device_batch_destroy(reinterpret_cast<DeviceBatch*>(db)); }
 
 gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut, int32_t* out_label, uint64_t* out_nodes,

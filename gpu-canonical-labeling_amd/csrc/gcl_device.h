@@ -11,7 +11,8 @@ struct DeviceGraphResult {
     uint64_t automorphisms;
     uint64_t splitter_steps;
     uint64_t csr_bytes;
-    int32_t status;   // 0 ok, otherwise a gcl_status
+    uint64_t device_ns;  // wall time of the graph's search on the device
+    int32_t status;      // 0 ok, otherwise a gcl_status
     int32_t pad;
 };
 
@@ -34,5 +35,6 @@ int32_t device_batch_size(const DeviceBatch* db);
 
 int32_t device_count();
 uint64_t step_budget();
+uint32_t spec_min_n();
 
 }  // namespace gcl

@@ -1,0 +1,1647 @@
+// gcl_worker.h -- device search worker of libgcl (included by gcl_device.hip).
+//
+// One wave64 runs the exact traversal of the reference's sequential engine
+// (reference src/sequential.cpp:216-670) for one graph at a time, pulling
+// graph ids from a launch-wide counter (persistent kernel).  All 64 lanes
+// execute the search's control flow in lockstep; the data-parallel inner
+// work of every step runs across the lanes:
+//   * refinement (reference src/partition_refinement.cpp:150-485): neighbour
+//     counts and per-cell ranks with LDS atomics, in-cell regrouping by hole /
+//     outsider pairing, active-set and cell-mask updates by the owner lane of
+//     each split cell, and the order-dependent hash chain over split cells by
+//     lane ranks;
+//   * backtracking: restore of a per-level snapshot of the cell-boundary
+//     bitmap and a rebuild of the cell arrays (replaces reconstruct_at_level;
+//     the order of vertices inside a cell is never observed);
+//   * target cells: member list sorted by id through a vertex bitmap;
+//   * orbit pruning: lock-free union-find (union by minimum);
+//   * leaves: sort-free comparison of the current relabeled graph with the
+//     stored first / best certificates (sorted canonical rows).
+// Root speculation: a worker whose graph is large publishes the remaining
+// root children on a board; idle workers refine them ahead of time and the
+// owner adopts the results in the exact sequential order, so the traversal,
+// node counts and canonical form are unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gcl_device.h"
+#include "gcl_search.h"
+
+namespace gcl {
+namespace dev {
+
+typedef uint16_t ix_t;
+constexpr int kWave = 64;
+constexpr int kBoard = 64;          // concurrent speculating graphs per launch
+constexpr uint32_t kNoKey = 0u;
+
+struct LevelRec {
+    uint32_t tc_first, tc_len, cursor, arena_off;
+    int64_t last_num_aut;
+};
+
+__host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// ------------------------------------------------------------------ layout
+// Per-worker global scratch.
+struct Layout {
+    uint64_t snap, seq, first_seq, best_seq, lv, arena_mem, arena_rep, firstcode, canon_code, first_label,
+        best_label, first_keys, best_keys, first_row, best_row, subgroup, auts, hot16, hot32, total;
+    uint32_t levels;  // snapshot levels available
+};
+
+constexpr int kHot16 = 10;  // elem pos cend cfirst tmp sa sb tv tcl sv
+constexpr int kHot32 = 4;   // cnt hits c1 c2
+
+__host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, uint32_t ring, uint64_t arena,
+                                              uint32_t levels, bool hot_global) {
+    Layout L;
+    uint64_t o = 0;
+    const uint64_t n1 = (uint64_t)maxn + 2;
+    const uint64_t nw1 = (uint64_t)maxn / 64 + 2;
+    L.levels = levels;
+    L.snap = o; o = align256(o + 8 * nw1 * (uint64_t)levels);
+    L.seq = o; o = align256(o + 2 * n1);
+    L.first_seq = o; o = align256(o + 2 * n1);
+    L.best_seq = o; o = align256(o + 2 * n1);
+    L.lv = o; o = align256(o + sizeof(LevelRec) * n1);
+    L.arena_mem = o; o = align256(o + 2 * arena);
+    L.arena_rep = o; o = align256(o + 2 * arena);
+    L.firstcode = o; o = align256(o + 4 * n1);
+    L.canon_code = o; o = align256(o + 4 * n1);
+    L.first_label = o; o = align256(o + 2 * n1);
+    L.best_label = o; o = align256(o + 2 * n1);
+    L.first_keys = o; o = align256(o + 4 * (max_nnz + 1));
+    L.best_keys = o; o = align256(o + 4 * (max_nnz + 1));
+    L.first_row = o; o = align256(o + 4 * n1);
+    L.best_row = o; o = align256(o + 4 * n1);
+    L.subgroup = o; o = align256(o + 4 * (uint64_t)ring);
+    L.auts = o; o = align256(o + 2 * (uint64_t)ring * maxn);
+    L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * n1);
+    L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (n1 + 1));
+    L.total = o;
+    return L;
+}
+
+// Speculation record: one per board slot.
+struct SpecHeader {
+    uint32_t epoch;     // publication number (item states carry it)
+    uint32_t graph;     // graph id of the publication
+    uint32_t count;     // items published
+    uint32_t next;      // helper claim counter
+    uint32_t tc_first;  // first position of the root target cell
+    uint32_t n;
+    uint32_t pad[2];
+};
+
+struct SpecLayout {
+    uint64_t header, root_elem, root_lasts, vert, state, code, ncells, steps, elem, lasts, total;
+    uint32_t slots;
+};
+
+__host__ __device__ inline SpecLayout make_spec_layout(uint32_t maxn, uint32_t slots) {
+    SpecLayout L;
+    const uint64_t n1 = (uint64_t)maxn + 2;
+    const uint64_t nw1 = (uint64_t)maxn / 64 + 2;
+    uint64_t o = 0;
+    L.slots = slots;
+    L.header = o; o = align256(o + sizeof(SpecHeader));
+    L.root_elem = o; o = align256(o + 2 * n1);
+    L.root_lasts = o; o = align256(o + 8 * nw1);
+    L.vert = o; o = align256(o + 2 * (uint64_t)slots);
+    L.state = o; o = align256(o + 4 * (uint64_t)slots);
+    L.code = o; o = align256(o + 4 * (uint64_t)slots);
+    L.ncells = o; o = align256(o + 4 * (uint64_t)slots);
+    L.steps = o; o = align256(o + 8 * (uint64_t)slots);
+    L.elem = o; o = align256(o + 2 * n1 * slots);
+    L.lasts = o; o = align256(o + 8 * nw1 * slots);
+    L.total = o;
+    return L;
+}
+
+struct KParams {
+    int32_t num_items;
+    const int32_t* items;       // graph ids processed by this launch
+    const int64_t* vtx_off;     // G+1
+    const int64_t* row_ptr;     // total_n+1 (global edge offsets)
+    const int32_t* col;         // local ids
+    const int32_t* init_order;  // per vertex: initial order for coloured graphs, or NULL
+    const int32_t* color;       // per vertex or NULL
+    int32_t* out_label;
+    DeviceGraphResult* out;
+    unsigned int* next_item;
+    uint8_t* scratch;
+    uint64_t scratch_stride;
+    uint32_t maxn;
+    uint64_t max_nnz;
+    uint32_t ring;
+    uint32_t max_aut;
+    uint64_t arena_cap;
+    uint32_t levels;
+    uint64_t step_budget;
+    // root speculation
+    uint8_t* spec;              // kBoard records
+    uint64_t spec_stride;
+    uint32_t spec_slots;
+    uint32_t spec_min_n;        // 0 disables speculation
+    unsigned int* board;        // kBoard words: 0 free, 1 owned, 2 published
+    unsigned int* pending;      // running graphs that may still publish
+};
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+// ordering of LDS traffic inside the one wave of a workgroup
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// ordering of global-memory traffic between lanes
+__device__ __forceinline__ void group_sync() { __syncthreads(); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, kWave));
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, (uint32_t)__shfl_xor(x, o, kWave));
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {  // inclusive
+    const int l = lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, kWave);
+        if (l >= o) x += y;
+    }
+    return x;
+}
+__device__ __forceinline__ uint32_t readlane(uint32_t v, int i) { return __builtin_amdgcn_readlane(v, i); }
+__device__ __forceinline__ uint32_t agent_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void agent_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool agent_cas(uint32_t* p, uint32_t expect, uint32_t desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ----------------------------------------------------------- hot storage
+template <int MAXN>
+struct LdsHot {
+    static constexpr int NW = MAXN / 64 + 2;
+    ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN], sa[MAXN], sb[MAXN], tv[MAXN], tcl[MAXN],
+        sv[MAXN];
+    uint32_t cnt[MAXN], hits[MAXN], c1[MAXN], c2[MAXN];
+    uint32_t rowl[MAXN + 1];
+    uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
+    uint32_t ctr[4];
+};
+
+// Hot state in LDS: addresses are compile-time constants.
+template <int MAXN>
+struct HotL {
+    static constexpr bool kLds = true;
+    __device__ __forceinline__ static LdsHot<MAXN>& S() {
+        __shared__ LdsHot<MAXN> s;
+        return s;
+    }
+    __device__ __forceinline__ ix_t& elem(uint32_t i) { return S().elem[i]; }
+    __device__ __forceinline__ ix_t& pos(uint32_t i) { return S().pos[i]; }
+    __device__ __forceinline__ ix_t& cend(uint32_t i) { return S().cend[i]; }
+    __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return S().cfirst[i]; }
+    __device__ __forceinline__ ix_t& tmp(uint32_t i) { return S().tmp[i]; }
+    __device__ __forceinline__ ix_t& sa(uint32_t i) { return S().sa[i]; }
+    __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().sb[i]; }
+    __device__ __forceinline__ ix_t& tv(uint32_t i) { return S().tv[i]; }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return S().tcl[i]; }
+    __device__ __forceinline__ ix_t& sv(uint32_t i) { return S().sv[i]; }
+    __device__ __forceinline__ uint32_t& cnt(uint32_t i) { return S().cnt[i]; }
+    __device__ __forceinline__ uint32_t& hits(uint32_t i) { return S().hits[i]; }
+    __device__ __forceinline__ uint32_t& c1(uint32_t i) { return S().c1[i]; }
+    __device__ __forceinline__ uint32_t& c2(uint32_t i) { return S().c2[i]; }
+    __device__ __forceinline__ uint64_t* active() { return S().active; }
+    __device__ __forceinline__ uint64_t* nsmask() { return S().nsmask; }
+    __device__ __forceinline__ uint64_t* vmask() { return S().vmask; }
+    __device__ __forceinline__ uint64_t* lasts() { return S().lasts; }
+    __device__ __forceinline__ uint32_t* ctr() { return S().ctr; }
+    __device__ __forceinline__ uint32_t row(uint32_t v) { return S().rowl[v]; }
+    __device__ __forceinline__ void sync() { wave_sync(); }
+    __device__ __forceinline__ void bind(const int64_t* rowp, uint32_t n, uint8_t*, uint32_t) {
+        for (uint32_t i = (uint32_t)lane_id(); i <= n; i += kWave) S().rowl[i] = (uint32_t)(rowp[i] - rowp[0]);
+    }
+};
+
+struct LdsBits {
+    uint64_t active[1026], nsmask[1026], vmask[1026], lasts[1026];
+    uint32_t ctr[4];
+};
+
+// Hot state in per-worker global scratch (graphs above the LDS classes);
+// the bitmaps stay in LDS.
+struct HotG {
+    static constexpr bool kLds = false;
+    ix_t* h16;
+    uint32_t* h32;
+    uint64_t n1;
+    const int64_t* rowp;
+    __device__ __forceinline__ static LdsBits& B() {
+        __shared__ LdsBits b;
+        return b;
+    }
+    __device__ __forceinline__ ix_t& elem(uint32_t i) { return h16[i]; }
+    __device__ __forceinline__ ix_t& pos(uint32_t i) { return h16[n1 + i]; }
+    __device__ __forceinline__ ix_t& cend(uint32_t i) { return h16[2 * n1 + i]; }
+    __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return h16[3 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tmp(uint32_t i) { return h16[4 * n1 + i]; }
+    __device__ __forceinline__ ix_t& sa(uint32_t i) { return h16[5 * n1 + i]; }
+    __device__ __forceinline__ ix_t& sb(uint32_t i) { return h16[6 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[7 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[8 * n1 + i]; }
+    __device__ __forceinline__ ix_t& sv(uint32_t i) { return h16[9 * n1 + i]; }
+    __device__ __forceinline__ uint32_t& cnt(uint32_t i) { return h32[i]; }
+    __device__ __forceinline__ uint32_t& hits(uint32_t i) { return h32[(n1 + 1) + i]; }
+    __device__ __forceinline__ uint32_t& c1(uint32_t i) { return h32[2 * (n1 + 1) + i]; }
+    __device__ __forceinline__ uint32_t& c2(uint32_t i) { return h32[3 * (n1 + 1) + i]; }
+    __device__ __forceinline__ uint64_t* active() { return B().active; }
+    __device__ __forceinline__ uint64_t* nsmask() { return B().nsmask; }
+    __device__ __forceinline__ uint64_t* vmask() { return B().vmask; }
+    __device__ __forceinline__ uint64_t* lasts() { return B().lasts; }
+    __device__ __forceinline__ uint32_t* ctr() { return B().ctr; }
+    __device__ __forceinline__ uint32_t row(uint32_t v) { return (uint32_t)(rowp[v] - rowp[0]); }
+    __device__ __forceinline__ void sync() { group_sync(); }
+    __device__ __forceinline__ void bind(const int64_t* rp, uint32_t, uint8_t* base, uint32_t maxn) {
+        rowp = rp;
+        n1 = (uint64_t)maxn + 2;
+        (void)base;
+    }
+};
+
+// ------------------------------------------------------------------ worker
+template <class Hot>
+struct Worker {
+    Hot H;
+    // graph
+    uint32_t n, nw;
+    uint64_t nnz;
+    const int64_t* rowp;  // global row offsets of the graph (n+1)
+    const int32_t* col;   // the graph's neighbour ids (local)
+    uint32_t ncells, nactive;
+    // per-worker global scratch
+    uint64_t* snap;
+    ix_t *seq, *first_seq, *best_seq, *arena_mem, *arena_rep, *first_label, *best_label, *auts;
+    LevelRec* lv;
+    int32_t *firstcode, *canon_code;
+    uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
+    uint64_t arena_cap;
+    uint32_t ring, levels;
+    uint64_t step_budget;
+    // search state (reference src/sequential.h:24-58)
+    uint32_t level, seq_size, tc_level_reached, eqlevfirst, first_seq_size, best_seq_size;
+    int eqlevcanon, comp_canon, canon_level, code;
+    bool first_discovered;
+    int64_t num_aut;
+    uint64_t nodes, steps, csr_bytes, budget;
+    int32_t status;
+    // speculation (owner side)
+    int32_t spec_slot;       // board slot published by this worker, or -1
+    bool spec_pending;       // counted in *pending until it publishes or ends
+    uint32_t spec_epoch, spec_cursor, spec_count;
+    uint8_t* spec_rec;
+    SpecLayout SL;
+
+    // --------------------------------------------------------- bit helpers
+    __device__ __forceinline__ static bool test_bit(const uint64_t* m, uint32_t p) {
+        return (m[p >> 6] >> (p & 63)) & 1ull;
+    }
+    __device__ __forceinline__ static void set_bit_atomic(uint64_t* m, uint32_t p) {
+        atomicOr((unsigned long long*)&m[p >> 6], (unsigned long long)(1ull << (p & 63)));
+    }
+    __device__ __forceinline__ static void clear_bit_atomic(uint64_t* m, uint32_t p) {
+        atomicAnd((unsigned long long*)&m[p >> 6], ~(unsigned long long)(1ull << (p & 63)));
+    }
+    // uniform single-bit update by lane 0
+    __device__ __forceinline__ void set_bit_u(uint64_t* m, uint32_t p, bool on) {
+        if (lane_id() == 0) {
+            if (on) m[p >> 6] |= 1ull << (p & 63);
+            else m[p >> 6] &= ~(1ull << (p & 63));
+        }
+    }
+    __device__ __forceinline__ void add_active(uint32_t p) {
+        if (p < n && !test_bit(H.active(), p)) {
+            set_bit_u(H.active(), p, true);
+            ++nactive;
+        }
+    }
+    __device__ __forceinline__ void remove_active(uint32_t p) {
+        if (p < n && test_bit(H.active(), p)) {
+            set_bit_u(H.active(), p, false);
+            --nactive;
+        }
+    }
+    // first set bit at position >= start, or -1
+    __device__ __forceinline__ int32_t first_set_from(const uint64_t* m, uint32_t start) const {
+        const int l = lane_id();
+        for (uint32_t base = start >> 6; base < nw; base += kWave) {
+            const uint32_t w = base + (uint32_t)l;
+            uint64_t bits = (w < nw) ? m[w] : 0ull;
+            if (w == (start >> 6)) bits &= ~0ull << (start & 63);
+            const uint64_t b = ballot(bits != 0ull);
+            if (b) {
+                const int fl = __builtin_ctzll(b);
+                const uint32_t lo = __shfl((uint32_t)bits, fl, kWave);
+                const uint32_t hi = __shfl((uint32_t)(bits >> 32), fl, kWave);
+                const uint64_t word = ((uint64_t)hi << 32) | lo;
+                return (int32_t)((base + (uint32_t)fl) * 64u + (uint32_t)__builtin_ctzll(word));
+            }
+        }
+        return -1;
+    }
+
+    // ------------------------------------------------------- cell rebuild
+    // Rebuilds cend / cfirst / nsmask / ncells from the cell-boundary bitmap
+    // `lasts` (bit p: p is the last position of a cell).
+    __device__ void rebuild_from_lasts() {
+        const int l = lane_id();
+        uint64_t* lasts = H.lasts();
+        uint32_t carry = 0xFFFFu;
+        uint32_t total = 0;
+        const uint32_t nchunks = (nw + kWave - 1) / kWave;
+        for (int32_t ch = (int32_t)nchunks - 1; ch >= 0; --ch) {
+            const uint32_t w = (uint32_t)ch * kWave + (uint32_t)l;
+            const uint64_t bits = (w < nw) ? lasts[w] : 0ull;
+            total += wave_sum((uint32_t)__popcll(bits));
+            uint32_t v = bits ? w : 0xFFFFu;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_down(v, o, kWave);
+                if (l + o < kWave) v = min(v, y);
+            }
+            v = min(v, carry);
+            if (w < nw) H.tmp(w) = (ix_t)v;  // next nonzero word at or after w
+            carry = __shfl(v, 0, kWave);
+        }
+        H.sync();
+        for (uint32_t base = 0; base < n; base += kWave) {
+            const uint32_t p = base + (uint32_t)l;
+            if (p < n) {
+                const uint32_t w = p >> 6;
+                const uint64_t bits = lasts[w] >> (p & 63);
+                uint32_t last;
+                if (bits) {
+                    last = p + (uint32_t)__builtin_ctzll(bits);
+                } else {
+                    const uint32_t w2 = H.tmp(w + 1);  // the last position is always a cell last
+                    last = w2 * 64u + (uint32_t)__builtin_ctzll(lasts[w2]);
+                }
+                H.cend(p) = (ix_t)last;
+                if (p == 0 || test_bit(lasts, p - 1)) H.cfirst(last) = (ix_t)p;
+            }
+        }
+        for (uint32_t base = 0; base < nw; base += kWave) {
+            const uint32_t w = base + (uint32_t)l;
+            if (w < nw) {
+                const uint64_t ls = lasts[w];
+                const uint64_t prev = (w == 0) ? 1ull : (lasts[w - 1] >> 63);
+                const uint64_t firsts = (ls << 1) | prev;  // p first <=> p-1 last, or p == 0
+                const uint64_t valid = (w * 64u + 64u <= n) ? ~0ull : ((1ull << (n - w * 64u)) - 1ull);
+                H.nsmask()[w] = (firsts & ~ls) & valid;
+            }
+        }
+        ncells = total;
+        H.sync();
+    }
+
+    __device__ void positions_from_elem() {
+        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.pos(H.elem(p)) = (ix_t)p;
+        H.sync();
+    }
+
+    __device__ void snapshot(uint32_t lev) {
+        // cell boundaries of the level-`lev` node, for backtracking
+        if (lev >= levels) {
+            status = GCL_E_CAPACITY;
+            return;
+        }
+        uint64_t* dst = snap + (uint64_t)lev * (nw + 1);
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) dst[w] = H.lasts()[w];
+    }
+
+    // --------------------------------------------------------- partitions
+    __device__ void init_partition(const int32_t* init_order, const int32_t* color, int64_t vbase) {
+        // initializePartition_s (reference src/partition_refinement.cpp:28-68):
+        // vertices in (colour, id) order, one cell per colour run.
+        const int l = lane_id();
+        for (uint32_t base = 0; base < n; base += kWave) {
+            const uint32_t p = base + (uint32_t)l;
+            bool is_last = false;
+            if (p < n) {
+                const uint32_t v = init_order ? (uint32_t)init_order[vbase + p] : p;
+                H.elem(p) = (ix_t)v;
+                if (p + 1 == n) is_last = true;
+                else if (color) {
+                    const uint32_t v2 = init_order ? (uint32_t)init_order[vbase + p + 1] : p + 1;
+                    is_last = color[vbase + v] != color[vbase + v2];
+                }
+                H.cnt(p) = 0;
+                H.hits(p) = 0;
+                H.c1(p) = 0;
+                H.c2(p) = 0;
+                H.sa(p) = 0;
+            }
+            const uint64_t b = ballot(is_last);
+            if (l == 0) H.lasts()[base >> 6] = b;
+        }
+        for (uint32_t w = (uint32_t)l; w < nw; w += kWave) {
+            H.active()[w] = 0;
+            H.vmask()[w] = 0;
+        }
+        nactive = 0;
+        H.sync();
+        positions_from_elem();
+        rebuild_from_lasts();
+    }
+
+    // Refinement step, splitter {w} with deg(w) <= 64 (reference
+    // src/partition_refinement.cpp:253-468, splitter.length == 1): cells
+    // holding neighbours split into (neighbours, others).
+    __device__ void singleton_step(uint32_t w, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        const uint32_t e0 = H.row(w);
+        const uint32_t deg = H.row(w + 1) - e0;
+        csr_bytes += 12 + 4ull * deg;
+        uint32_t u = 0, c = 0, f = 0, r = 0;
+        bool ns = false;
+        if ((uint32_t)l < deg) {
+            u = (uint32_t)col[e0 + (uint32_t)l];
+            c = H.cend(H.pos(u));
+            f = H.cfirst(c);
+            ns = f != c;
+        }
+        if (ns) {
+            H.cnt(u) = 1;
+            r = atomicAdd(&H.hits(c), 1u);
+        }
+        H.sync();
+        uint32_t k = 0, L = 0, q = 0;
+        bool split = false, out = false;
+        if (ns) {
+            k = H.hits(c);
+            L = c - f + 1;
+            split = k < L;
+            q = H.pos(u);
+            out = split && q >= f + k;
+        }
+        if (split) {
+            const uint32_t x = H.elem(f + r);
+            if (H.cnt(x) == 0) {
+                const uint32_t h = atomicAdd(&H.c1(c), 1u);
+                H.tmp(f + h) = (ix_t)(f + r);
+            }
+        }
+        uint32_t o = 0;
+        if (out) o = atomicAdd(&H.c2(c), 1u);
+        H.sync();
+        if (out) {  // move u into the front block over a non-neighbour
+            const uint32_t hp = H.tmp(f + o);
+            const uint32_t x = H.elem(hp);
+            H.elem(hp) = (ix_t)u;
+            H.pos(u) = (ix_t)hp;
+            H.elem(q) = (ix_t)x;
+            H.pos(x) = (ix_t)q;
+        }
+        H.sync();
+        const bool owner = ns && r == 0;
+        const bool own = owner && split;
+        if (split) H.cend(f + r) = (ix_t)(f + k - 1);
+        uint32_t hkey = kNoKey;
+        if (owner) {
+            H.hits(c) = 0;
+            H.c1(c) = 0;
+            H.c2(c) = 0;
+        }
+        if (own) {
+            H.cfirst(f + k - 1) = (ix_t)f;
+            H.cfirst(c) = (ix_t)(f + k);
+            // active-set rule (414-435): one new active cell per split
+            uint32_t cand = 0xFFFFu;
+            if (test_bit(H.active(), f) || k >= L - k) {
+                set_bit_atomic(H.active(), f + k);
+                if (L - k == 1) cand = f + k;
+            } else {
+                set_bit_atomic(H.active(), f);
+                if (k == 1) cand = f;
+            }
+            if (cand != 0xFFFFu) hkey = ((f + 1) << 16) | cand;
+            if (k == 1) clear_bit_atomic(H.nsmask(), f);
+            if (L - k > 1) set_bit_atomic(H.nsmask(), f + k);
+            set_bit_atomic(H.lasts(), f + k - 1);
+        }
+        if (ns) H.cnt(u) = 0;
+        const uint64_t om = ballot(own);
+        const uint32_t nsplit = (uint32_t)__popcll(om);
+        if (nsplit == 0) {
+            H.sync();
+            return;
+        }
+        ncells += nsplit;
+        nactive += nsplit;
+        // hint: the last split in position order that sets one
+        const uint32_t hk = wave_max(hkey);
+        if (hk != kNoKey) hint = (int)(hk & 0xFFFFu);
+        // boundary hash chain in position order (399-409): x = f+k-1 grows with f
+        const uint32_t x = f + k - 1;
+        uint32_t rank = 0;
+        for (uint64_t m = om; m; m &= m - 1) {
+            const uint32_t xj = readlane(x, __builtin_ctzll(m));
+            if (own && xj < x) ++rank;
+        }
+        for (uint32_t t = 0; t < nsplit; ++t) {
+            const uint64_t b = ballot(own && rank == t);
+            longcode = GCL_MASH(longcode, readlane(x, __builtin_ctzll(b)));
+        }
+        H.sync();
+    }
+
+    // General refinement step: splitter cell [p, e] of any size (reference
+    // src/partition_refinement.cpp:197-473): counts with LDS atomics, every
+    // touched cell regrouped by a counting sort on its key (count for
+    // |W| > 1; neighbours first for |W| == 1), then one pass over the
+    // pre-step non-singleton cells in position order for the hash chain and
+    // the active-set rules.
+    __device__ void general_step(uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        const uint32_t s = e - p + 1;
+        uint32_t* ctr = H.ctr();
+        if (l == 0) {
+            ctr[0] = 0;
+            ctr[1] = 0;
+        }
+        H.sync();
+        uint32_t degsum = 0;
+        for (uint32_t base = 0; base < s; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < s) {
+                const uint32_t w = H.elem(p + i);
+                const uint32_t a = H.row(w), b = H.row(w + 1);
+                degsum += b - a;
+                for (uint32_t x = a; x < b; ++x) {
+                    const uint32_t u = (uint32_t)col[x];
+                    if (atomicAdd(&H.cnt(u), 1u) == 0u) {
+                        const uint32_t idx = atomicAdd(&ctr[0], 1u);
+                        H.tv(idx) = (ix_t)u;
+                    }
+                }
+            }
+        }
+        csr_bytes += 12ull * s + 4ull * wave_sum(degsum);
+        H.sync();
+        const uint32_t ntv = ctr[0];
+        for (uint32_t base = 0; base < ntv; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < ntv) {
+                const uint32_t u = H.tv(i);
+                const uint32_t c = H.cend(H.pos(u));
+                if (H.cfirst(c) != c) {
+                    if (atomicAdd(&H.hits(c), 1u) == 0u) {
+                        const uint32_t idx = atomicAdd(&ctr[1], 1u);
+                        H.tcl(idx) = (ix_t)c;
+                    }
+                }
+            }
+        }
+        H.sync();
+        const uint32_t ntc = ctr[1];
+        for (uint32_t j = 0; j < ntc; ++j) {
+            const uint32_t c = H.tcl(j);
+            const uint32_t f = H.cfirst(c);
+            const uint32_t L = c - f + 1;
+            uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
+            for (uint32_t base = 0; base < L; base += kWave) {
+                const uint32_t i = base + (uint32_t)l;
+                if (i < L) {
+                    const uint32_t v = H.elem(f + i);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    kmin = min(kmin, kv);
+                    kmax = max(kmax, kv);
+                }
+            }
+            kmin = wave_min(kmin);
+            kmax = wave_max(kmax);
+            if (l == 0) H.hits(c) = 0;
+            if (kmin == kmax) {
+                if (l == 0) {
+                    H.sa(f) = 1;
+                    H.sv(f) = (ix_t)((s == 1) ? 1u - kmin : kmin);
+                    H.tcl(j) = (ix_t)f;
+                }
+                H.sync();
+                continue;
+            }
+            const uint32_t R = kmax - kmin + 1;
+            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.c1(b) = 0;
+            H.sync();
+            for (uint32_t base = 0; base < L; base += kWave) {
+                const uint32_t i = base + (uint32_t)l;
+                if (i < L) {
+                    const uint32_t v = H.elem(f + i);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    H.sb(v) = (ix_t)atomicAdd(&H.c1(kv - kmin), 1u);
+                }
+            }
+            H.sync();
+            uint32_t run = 0, nfrag = 0;
+            for (uint32_t base = 0; base < R; base += kWave) {
+                const uint32_t b = base + (uint32_t)l;
+                const uint32_t sz = b < R ? H.c1(b) : 0u;
+                const uint32_t inc = wave_scan(sz);
+                if (b < R) H.c2(b) = run + inc - sz;
+                run += __shfl(inc, kWave - 1, kWave);
+                nfrag += (uint32_t)__popcll(ballot(sz != 0));
+            }
+            H.sync();
+            for (uint32_t base = 0; base < L; base += kWave) {
+                const uint32_t i = base + (uint32_t)l;
+                if (i < L) {
+                    const uint32_t v = H.elem(f + i);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    H.tmp(f + H.c2(kv - kmin) + H.sb(v)) = (ix_t)v;
+                }
+            }
+            H.sync();
+            for (uint32_t base = 0; base < L; base += kWave) {
+                const uint32_t i = base + (uint32_t)l;
+                if (i < L) {
+                    const uint32_t P = f + i;
+                    const uint32_t v = H.tmp(P);
+                    H.elem(P) = (ix_t)v;
+                    H.pos(v) = (ix_t)P;
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    const uint32_t b = kv - kmin;
+                    const uint32_t first = f + H.c2(b);
+                    const uint32_t last = first + H.c1(b) - 1;
+                    H.cend(P) = (ix_t)last;
+                    if (P == first) {
+                        H.cfirst(last) = (ix_t)first;
+                        if (last != c) set_bit_atomic(H.lasts(), last);
+                    }
+                }
+            }
+            ncells += nfrag - 1;
+            H.sync();
+            for (uint32_t b = (uint32_t)l; b < R; b += kWave) {
+                H.c1(b) = 0;
+                H.c2(b) = 0;
+            }
+            if (l == 0) {
+                H.sa(f) = 2;
+                H.sv(f) = (ix_t)c;
+                H.tcl(j) = (ix_t)f;
+            }
+            H.sync();
+        }
+        // hash chain / active rules over the pre-step non-singleton cells
+        const uint64_t* nsm = H.nsmask();
+        for (uint32_t wi = 0; wi < nw; ++wi) {
+            uint64_t bits = nsm[wi];
+            while (bits) {
+                const uint32_t f = wi * 64u + (uint32_t)__builtin_ctzll(bits);
+                bits &= bits - 1;
+                const uint32_t st = H.sa(f);
+                if (st == 0) {
+                    if (s > 1) longcode = GCL_MASH(longcode, f);
+                    continue;
+                }
+                if (st == 1) {
+                    if (s > 1) longcode = GCL_MASH(longcode, (uint32_t)H.sv(f) + f);
+                    continue;
+                }
+                const uint32_t cold = H.sv(f);
+                if (s == 1) {
+                    const uint32_t K = (uint32_t)H.cend(f) - f + 1;
+                    const uint32_t LL = cold - f + 1;
+                    longcode = GCL_MASH(longcode, f + K - 1);
+                    if (test_bit(H.active(), f) || K >= LL - K) {
+                        add_active(f + K);
+                        if (LL - K == 1) hint = (int)(f + K);
+                    } else {
+                        add_active(f);
+                        if (K == 1) hint = (int)f;
+                    }
+                    continue;
+                }
+                uint32_t max_size = 0, max_first = f;
+                for (uint32_t q = f; q <= cold;) {
+                    const uint32_t last = H.cend(q);
+                    const uint32_t size = last - q + 1;
+                    longcode = GCL_MASH(longcode, H.cnt(H.elem(q)) + q);
+                    if (size > max_size) {
+                        max_size = size;
+                        max_first = q;
+                    }
+                    q = last + 1;
+                }
+                const bool was_active = test_bit(H.active(), f);
+                for (uint32_t q = (uint32_t)H.cend(f) + 1; q <= cold;) {
+                    const uint32_t last = H.cend(q);
+                    add_active(q);
+                    if (last == q) hint = (int)q;
+                    q = last + 1;
+                }
+                if (!was_active) {
+                    add_active(f);
+                    remove_active(max_first);
+                }
+            }
+        }
+        H.sync();
+        // cell masks of the new fragments; reset per-cell state
+        for (uint32_t base = 0; base < ntc; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < ntc) {
+                const uint32_t f = H.tcl(i);
+                if (H.sa(f) == 2) {
+                    const uint32_t cold = H.sv(f);
+                    for (uint32_t q = f; q <= cold;) {
+                        const uint32_t last = H.cend(q);
+                        if (last > q) set_bit_atomic(H.nsmask(), q);
+                        else clear_bit_atomic(H.nsmask(), q);
+                        q = last + 1;
+                    }
+                }
+                H.sa(f) = 0;
+            }
+        }
+        H.sync();
+        for (uint32_t base = 0; base < ntv; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < ntv) H.cnt(H.tv(i)) = 0;
+        }
+        H.sync();
+    }
+
+    // Partition_s_refinement (reference src/partition_refinement.cpp:150-485)
+    __device__ int refine(int32_t init_active) {
+        const int l = lane_id();
+        if (init_active < 0) {
+            uint32_t total = 0;
+            for (uint32_t base = 0; base < n; base += kWave) {
+                const uint32_t p = base + (uint32_t)l;
+                const bool first = p < n && H.cfirst(H.cend(p)) == p;
+                const uint64_t b = ballot(first);
+                if (l == 0) H.active()[base >> 6] = b;
+                total += (uint32_t)__popcll(b);
+            }
+            nactive = total;
+        } else {
+            add_active((uint32_t)init_active);
+        }
+        H.sync();
+        uint32_t longcode = ncells;
+        int hint = 0;
+        uint64_t bud = budget;
+        while (ncells < n && nactive > 0) {
+            if (bud == 0) {
+                status = GCL_E_INTERNAL;  // step budget exhausted
+                break;
+            }
+            --bud;
+            int32_t sp = first_set_from(H.active(), (uint32_t)hint);
+            if (sp < 0) sp = first_set_from(H.active(), 0);
+            const uint32_t p = (uint32_t)sp;
+            set_bit_u(H.active(), p, false);
+            --nactive;
+            ++steps;
+            const uint32_t e = H.cend(p);
+            longcode = GCL_MASH(longcode, p + e);
+            if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
+            H.sync();
+            if (e == p) {
+                const uint32_t w = H.elem(p);
+                if (H.row(w + 1) - H.row(w) <= (uint32_t)kWave) singleton_step(w, longcode, hint);
+                else general_step(p, e, longcode, hint);
+            } else {
+                general_step(p, e, longcode, hint);
+            }
+        }
+        budget = bud;
+        if (nactive) {
+            for (uint32_t w = (uint32_t)l; w < nw; w += kWave) H.active()[w] = 0;
+            nactive = 0;
+            H.sync();
+        }
+        longcode = GCL_MASH(longcode, ncells);
+        return GCL_CLEANUP(longcode);
+    }
+
+    // split_by_and_refine (reference src/partition_refinement.cpp:486-539)
+    __device__ void individualize_and_refine(uint32_t v, uint32_t f) {
+        const uint32_t c = H.cend(f);
+        const uint32_t q = H.pos(v);
+        const uint32_t x = H.elem(f);
+        H.sync();
+        if (lane_id() == 0) {
+            H.elem(f) = (ix_t)v;
+            H.pos(v) = (ix_t)f;
+            H.elem(q) = (ix_t)x;
+            H.pos(x) = (ix_t)q;
+            H.cend(f) = (ix_t)f;
+            H.cfirst(f) = (ix_t)f;
+            H.cfirst(c) = (ix_t)(f + 1);
+        }
+        set_bit_u(H.nsmask(), f, false);
+        set_bit_u(H.nsmask(), f + 1, c > f + 1);
+        set_bit_u(H.lasts(), f, true);
+        ++ncells;
+        H.sync();
+        code = refine((int32_t)f);
+    }
+
+    // Backtrack the partition to its state at `lev` (reconstruct_at_level,
+    // reference src/partition_refinement.cpp:541-604, up to in-cell order).
+    __device__ void restore_level(uint32_t lev) {
+        const uint64_t* src = snap + (uint64_t)lev * (nw + 1);
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = src[w];
+        H.sync();
+        rebuild_from_lasts();
+    }
+
+    // --------------------------------------------------------- search ops
+    // target_cell_selector with hint -1 (reference
+    // src/partition_refinement.cpp:607-655): first non-singleton cell, members
+    // in increasing vertex id.
+    __device__ bool select_target() {
+        const int32_t f = first_set_from(H.nsmask(), 0);
+        if (f < 0) return true;
+        const int l = lane_id();
+        const uint32_t c = H.cend((uint32_t)f);
+        const uint32_t t = c - (uint32_t)f + 1;
+        const uint32_t idx = tc_level_reached;
+        const uint64_t off = idx == 0 ? 0 : (uint64_t)lv[idx - 1].arena_off + lv[idx - 1].tc_len;
+        if (off + t > arena_cap) {
+            status = GCL_E_CAPACITY;
+            return false;
+        }
+        for (uint32_t i = (uint32_t)l; i < t; i += kWave) set_bit_atomic(H.vmask(), H.elem((uint32_t)f + i));
+        H.sync();
+        uint32_t run = 0;
+        for (uint32_t base = 0; base < nw; base += kWave) {
+            const uint32_t w = base + (uint32_t)l;
+            uint64_t bits = w < nw ? H.vmask()[w] : 0ull;
+            const uint32_t c0 = (uint32_t)__popcll(bits);
+            const uint32_t inc = wave_scan(c0);
+            uint32_t j = run + inc - c0;
+            while (bits) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(bits);
+                bits &= bits - 1;
+                arena_mem[off + j] = (ix_t)(w * 64u + b);
+                arena_rep[off + j] = (ix_t)(w * 64u + b);
+                ++j;
+            }
+            if (w < nw) H.vmask()[w] = 0;
+            run += __shfl(inc, kWave - 1, kWave);
+        }
+        LevelRec r;
+        r.tc_first = (uint32_t)f;
+        r.tc_len = t;
+        r.cursor = 0;
+        r.arena_off = (uint32_t)off;
+        r.last_num_aut = 0;
+        lv[idx] = r;  // uniform store by every lane
+        ++tc_level_reached;
+        group_sync();
+        return true;
+    }
+
+    // subgroup_fixing_sequence + compute_orbits_fast (reference
+    // src/perm_group.cpp:5-116), union by minimum without locks.
+    __device__ void compute_orbits(uint32_t li) {
+        const int l = lane_id();
+        const LevelRec r = lv[li];
+        const int64_t upto = min(num_aut, (int64_t)ring);
+        const int64_t from = r.last_num_aut < 0 ? 0 : r.last_num_aut;
+        uint32_t nsub = 0;
+        for (int64_t i = from; i < upto; ++i) {
+            const ix_t* a = auts + (uint64_t)i * n;
+            bool bad = false;
+            for (uint32_t base = 0; base < seq_size; base += kWave) {
+                const uint32_t j = base + (uint32_t)l;
+                const bool b = j < seq_size && a[seq[j]] != seq[j];
+                if (ballot(b)) {
+                    bad = true;
+                    break;
+                }
+            }
+            if (!bad) {
+                subgroup[nsub] = (uint32_t)i;  // uniform store
+                ++nsub;
+            }
+        }
+        if (nsub == 0) return;
+        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt(v) = v;  // cnt is zero between steps
+        H.sync();
+        for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave)
+            H.cnt(arena_mem[r.arena_off + i]) = arena_rep[r.arena_off + i];
+        H.sync();
+        for (uint32_t s = 0; s < nsub; ++s) {
+            const ix_t* a = auts + (uint64_t)subgroup[s] * n;
+            for (uint32_t v = (uint32_t)l; v < n; v += kWave) {
+                const uint32_t img = a[v];
+                if (img == v) continue;
+                // parents only point to smaller ids, so walks are shorter
+                // than n; the caps only bound a corrupted structure
+                uint32_t x = v, y = img;
+                for (uint32_t attempt = 0; attempt < n; ++attempt) {
+                    for (uint32_t h = 0; h < n; ++h) {
+                        const uint32_t px = __hip_atomic_load(&H.cnt(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (px == x) break;
+                        x = px;
+                    }
+                    for (uint32_t h = 0; h < n; ++h) {
+                        const uint32_t py = __hip_atomic_load(&H.cnt(y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (py == y) break;
+                        y = py;
+                    }
+                    if (x == y) break;
+                    const uint32_t lo = min(x, y), hi = max(x, y);
+                    if (atomicCAS(&H.cnt(hi), hi, lo) == hi) break;
+                    x = lo;
+                    y = hi;
+                }
+            }
+            H.sync();
+        }
+        for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave) {
+            uint32_t x = arena_mem[r.arena_off + i];
+            for (uint32_t h = 0; h < n && H.cnt(x) != x; ++h) x = H.cnt(x);
+            arena_rep[r.arena_off + i] = (ix_t)x;
+        }
+        H.sync();
+        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt(v) = 0;
+        group_sync();
+    }
+
+    // Certificate of the current leaf: sorted canonical rows (keys i*n+j)
+    // with row offsets, the sparse form of the reference's n x n leaf hash
+    // (src/CSRGraph.cpp:332-351).
+    __device__ void store_cert(uint32_t* keys, uint32_t* row, ix_t* label) {
+        const int l = lane_id();
+        uint32_t run = 0;
+        for (uint32_t base = 0; base < n; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            uint32_t v = 0, d = 0;
+            if (i < n) {
+                v = H.elem(i);
+                d = H.row(v + 1) - H.row(v);
+            }
+            const uint32_t inc = wave_scan(d);
+            const uint32_t o = run + inc - d;
+            if (i < n) {
+                row[i] = o;
+                label[i] = (ix_t)v;
+                const uint32_t a = H.row(v);
+                for (uint32_t t = 0; t < d; ++t) {
+                    const uint32_t key = i * n + H.pos((uint32_t)col[a + t]);
+                    uint32_t z = t;
+                    while (z > 0 && keys[o + z - 1] > key) {
+                        keys[o + z] = keys[o + z - 1];
+                        --z;
+                    }
+                    keys[o + z] = key;
+                }
+            }
+            run += __shfl(inc, kWave - 1, kWave);
+        }
+        if (l == 0) row[n] = run;
+        group_sync();
+    }
+
+    // compare_adj_upper (reference src/sequential.cpp:114-124) of the current
+    // leaf against a stored certificate, without sorting the current rows: at
+    // the first row whose neighbour sets differ, the graph holding the
+    // smallest element of the symmetric difference is the greater one.
+    __device__ int compare_cert(const uint32_t* keys, const uint32_t* row) {
+        const int l = lane_id();
+        for (uint32_t base = 0; base < n; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            bool differs = false;
+            if (i < n) {
+                const uint32_t v = H.elem(i);
+                const uint32_t a = H.row(v);
+                const uint32_t d = H.row(v + 1) - a;
+                const uint32_t r0 = row[i], r1 = row[i + 1];
+                if (r1 - r0 != d) {
+                    differs = true;
+                } else {
+                    for (uint32_t t = 0; t < d && !differs; ++t) {
+                        const uint32_t key = i * n + H.pos((uint32_t)col[a + t]);
+                        uint32_t lo = r0, hi = r1;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (keys[mid] < key) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        if (lo == r1 || keys[lo] != key) differs = true;
+                    }
+                }
+            }
+            const uint64_t dm = ballot(differs);
+            if (dm) {
+                const uint32_t i0 = base + (uint32_t)__builtin_ctzll(dm);
+                const uint32_t v = H.elem(i0);
+                const uint32_t a = H.row(v);
+                const uint32_t d = H.row(v + 1) - a;
+                const uint32_t r0 = row[i0], r1 = row[i0 + 1];
+                uint32_t min_cur = 0xFFFFFFFFu, min_ref = 0xFFFFFFFFu;
+                for (uint32_t t = (uint32_t)l; t < d; t += kWave) {
+                    const uint32_t key = i0 * n + H.pos((uint32_t)col[a + t]);
+                    bool found = false;
+                    for (uint32_t z = r0; z < r1; ++z)
+                        if (keys[z] == key) {
+                            found = true;
+                            break;
+                        }
+                    if (!found) min_cur = min(min_cur, key);
+                }
+                for (uint32_t z = r0 + (uint32_t)l; z < r1; z += kWave) {
+                    const uint32_t key = keys[z];
+                    bool found = false;
+                    for (uint32_t t = 0; t < d; ++t)
+                        if (i0 * n + H.pos((uint32_t)col[a + t]) == key) {
+                            found = true;
+                            break;
+                        }
+                    if (!found) min_ref = min(min_ref, key);
+                }
+                min_cur = wave_min(min_cur);
+                min_ref = wave_min(min_ref);
+                return min_cur < min_ref ? 1 : -1;
+            }
+        }
+        return 0;
+    }
+
+    __device__ void store_aut(const ix_t* ref_label) {
+        const uint64_t slot = (uint64_t)(num_aut % (int64_t)ring);
+        ix_t* dst = auts + slot * n;
+        for (uint32_t j = (uint32_t)lane_id(); j < n; j += kWave) dst[ref_label[j]] = H.elem(j);
+        ++num_aut;
+        group_sync();
+    }
+
+    __device__ int gca_level(const ix_t* a, uint32_t na) {
+        // get_gca_level (reference src/sequential.cpp:183-213)
+        const uint32_t m = min(na, seq_size);
+        const int l = lane_id();
+        for (uint32_t base = 0; base < m; base += kWave) {
+            const uint32_t k = base + (uint32_t)l;
+            const uint64_t dm = ballot(k < m && a[k] != seq[k]);
+            if (dm) return (int)(base + (uint32_t)__builtin_ctzll(dm)) + 1;
+        }
+        return -1;
+    }
+
+    __device__ void backtrack(uint32_t lev) {
+        // backtrack_to (reference src/sequential.cpp:650-670)
+        if (lev == 0) {
+            level = 0;
+            return;
+        }
+        restore_level(lev);
+        if (lev < eqlevfirst) eqlevfirst = lev;
+        if ((int)lev <= eqlevcanon) {
+            eqlevcanon = (int)lev;
+            comp_canon = 0;
+        }
+        seq_size = lev - 1;
+        tc_level_reached = lev;
+        level = lev;
+    }
+
+    __device__ void first_terminal() {
+        // first_terminal (reference src/sequential.cpp:259-303)
+        eqlevfirst = level;
+        firstcode[level] = kSentinel;
+        store_cert(first_keys, first_row, first_label);
+        store_cert(best_keys, best_row, best_label);
+        for (uint32_t i = (uint32_t)lane_id(); i < seq_size; i += kWave) {
+            first_seq[i] = seq[i];
+            best_seq[i] = seq[i];
+        }
+        first_seq_size = best_seq_size = seq_size;
+        first_discovered = true;
+        canon_level = eqlevcanon = (int)level;
+        comp_canon = 0;
+        for (uint32_t i = (uint32_t)lane_id(); i < level; i += kWave) canon_code[i] = firstcode[i];
+        group_sync();
+        canon_code[level] = kSentinel;
+        group_sync();
+    }
+
+    // descend from the current node (level `level`) into child v
+    __device__ void descend(uint32_t v, uint32_t tc_first) {
+        ++nodes;
+        seq[seq_size] = (ix_t)v;
+        ++seq_size;
+        group_sync();
+        individualize_and_refine(v, tc_first);
+        ++level;
+        snapshot(level);
+    }
+
+    __device__ bool first_path_node() {
+        // first_path_node (reference src/sequential.cpp:216-257)
+        if (ncells == n) {
+            first_terminal();
+            backtrack(level - 1);
+            return true;
+        }
+        if (!select_target()) return false;
+        LevelRec r = lv[tc_level_reached - 1];
+        const uint32_t v = arena_mem[r.arena_off + r.cursor];
+        lv[tc_level_reached - 1].cursor = r.cursor + 1;
+        descend(v, r.tc_first);
+        firstcode[level - 1] = code;
+        return true;
+    }
+
+    __device__ void classify() {
+        // classify_node (reference src/sequential.cpp:396-624)
+        const uint32_t L = level;
+        if (eqlevfirst == L - 1 && code == firstcode[L - 1]) eqlevfirst = L;
+        if (eqlevcanon == (int)L - 1) {
+            const int cc = canon_code[L - 1];
+            if (code < cc) comp_canon = -1;
+            else if (code > cc) comp_canon = 1;
+            else {
+                comp_canon = 0;
+                eqlevcanon = (int)L;
+            }
+        }
+        if (comp_canon > 0) canon_code[L - 1] = code;
+        int decision = kNormal;
+        if (eqlevfirst != L && comp_canon < 0) {
+            decision = kBad;
+        } else if (ncells == n) {
+            if (eqlevfirst == L && compare_cert(first_keys, first_row) == 0) decision = kAutFirst;
+            if (decision == kNormal) {
+                if (comp_canon == 0) {
+                    if ((int)L < canon_level) comp_canon = 1;
+                    else comp_canon = compare_cert(best_keys, best_row);
+                }
+                if (comp_canon == 0) decision = kAutBest;
+                else if (comp_canon > 0) decision = kNewBest;
+                else decision = kBad;
+            }
+        }
+        group_sync();
+        switch (decision) {
+            case kNormal:
+                break;
+            case kAutFirst: {
+                store_aut(first_label);
+                const int gl = gca_level(first_seq, first_seq_size);
+                backtrack(gl >= 0 ? (uint32_t)gl : L - 1);
+                break;
+            }
+            case kAutBest: {
+                store_aut(best_label);
+                const int gl = gca_level(best_seq, best_seq_size);
+                backtrack(gl >= 0 ? (uint32_t)gl : L - 1);
+                break;
+            }
+            case kNewBest: {
+                canon_level = eqlevcanon = (int)L;
+                comp_canon = 0;
+                canon_code[L] = kSentinel;
+                store_cert(best_keys, best_row, best_label);
+                for (uint32_t i = (uint32_t)lane_id(); i < seq_size; i += kWave) best_seq[i] = seq[i];
+                best_seq_size = seq_size;
+                group_sync();
+                backtrack(L - 1);
+                break;
+            }
+            case kBad:
+                backtrack(L - 1);
+                break;
+        }
+    }
+
+    // ------------------------------------------------- root speculation
+    __device__ SpecHeader* spec_header() { return (SpecHeader*)(spec_rec + SL.header); }
+    __device__ uint32_t* spec_state() { return (uint32_t*)(spec_rec + SL.state); }
+
+    // Publishes the remaining root children (orbit representatives at and
+    // after the cursor) for other workers to refine.  Called at level 1.
+    __device__ void spec_publish(const KParams& P, const LevelRec& r, uint32_t graph) {
+        if (spec_pending) {
+            spec_pending = false;
+            if (lane_id() == 0) atomicSub(P.pending, 1u);
+        }
+        // count candidates
+        uint32_t cnt = 0;
+        for (uint32_t i = r.cursor + (uint32_t)lane_id(); i < r.tc_len; i += kWave)
+            cnt += arena_rep[r.arena_off + i] == arena_mem[r.arena_off + i] ? 1u : 0u;
+        cnt = wave_sum(cnt);
+        if (cnt < 2) return;
+        // claim a board slot
+        int32_t slot = -1;
+        if (lane_id() == 0) {
+            for (int b = 0; b < kBoard; ++b)
+                if (agent_cas(&P.board[b], 0u, 1u)) {
+                    slot = b;
+                    break;
+                }
+        }
+        slot = __shfl(slot, 0, kWave);
+        if (slot < 0) return;
+        spec_slot = slot;
+        spec_rec = P.spec + (uint64_t)slot * P.spec_stride;
+        SL = make_spec_layout(P.maxn, P.spec_slots);
+        SpecHeader* hd = spec_header();
+        const uint32_t epoch = hd->epoch + 1;
+        spec_epoch = epoch;
+        const uint32_t count = min(cnt, SL.slots);
+        spec_count = count;
+        spec_cursor = 0;
+        // root partition and item list
+        ix_t* re = (ix_t*)(spec_rec + SL.root_elem);
+        uint64_t* rl = (uint64_t*)(spec_rec + SL.root_lasts);
+        const uint64_t* s1 = snap + 1ull * (nw + 1);
+        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) re[p] = H.elem(p);
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) rl[w] = s1[w];
+        ix_t* vert = (ix_t*)(spec_rec + SL.vert);
+        uint32_t run = 0;
+        for (uint32_t base = r.cursor; base < r.tc_len && run < count; base += kWave) {
+            const uint32_t i = base + (uint32_t)lane_id();
+            const bool ok = i < r.tc_len && arena_rep[r.arena_off + i] == arena_mem[r.arena_off + i];
+            const uint64_t m = ballot(ok);
+            const uint32_t j = run + lanes_below(m);
+            if (ok && j < count) vert[j] = arena_mem[r.arena_off + i];
+            run += (uint32_t)__popcll(m);
+        }
+        if (lane_id() == 0) {
+            hd->graph = graph;
+            hd->count = count;
+            hd->tc_first = r.tc_first;
+            hd->n = n;
+            __hip_atomic_store(&hd->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // data and header before the item states that make items claimable
+        __threadfence();
+        group_sync();
+        uint32_t* state = spec_state();
+        for (uint32_t j = (uint32_t)lane_id(); j < count; j += kWave) state[j] = epoch << 2;
+        __threadfence();
+        group_sync();
+        if (lane_id() == 0) {
+            agent_store(&hd->epoch, epoch);
+            agent_store(&P.board[slot], 2u);
+        }
+        group_sync();
+    }
+
+    // Root child v: adopt a helper's refinement, or claim and refine it here.
+    // Returns true when v was handled (partition at level 2, code set).
+    __device__ bool spec_take(uint32_t v) {
+        if (spec_slot < 0) return false;
+        const ix_t* vert = (const ix_t*)(spec_rec + SL.vert);
+        uint32_t* state = spec_state();
+        const uint32_t E = spec_epoch << 2;
+        // skip (and withdraw) published items that became non-representatives
+        while (spec_cursor < spec_count && (uint32_t)vert[spec_cursor] < v) {
+            if (lane_id() == 0) agent_cas(&state[spec_cursor], E | 0u, E | 3u);
+            ++spec_cursor;
+        }
+        if (spec_cursor >= spec_count || (uint32_t)vert[spec_cursor] != v) return false;
+        const uint32_t j = spec_cursor++;
+        uint32_t won = 0;
+        if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 1u) ? 1u : 0u;
+        won = __shfl(won, 0, kWave);
+        if (won) return false;  // refine it here (the item is withdrawn)
+        // a helper has it: wait until done, then copy its partition
+        for (uint64_t spin = 0;; ++spin) {
+            const uint32_t st = agent_load(&state[j]);
+            if (st == (E | 2u)) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
+        const uint64_t* il = (const uint64_t*)(spec_rec + SL.lasts) + (uint64_t)j * (P_nw1());
+        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = il[w];
+        code = (int)((const uint32_t*)(spec_rec + SL.code))[j];
+        steps += ((const uint64_t*)(spec_rec + SL.steps))[j];
+        H.sync();
+        positions_from_elem();
+        rebuild_from_lasts();
+        return true;
+    }
+    __device__ uint64_t P_n1() const { return (uint64_t)spec_maxn + 2; }
+    __device__ uint64_t P_nw1() const { return (uint64_t)spec_maxn / 64 + 2; }
+    uint32_t spec_maxn;
+
+    // Ends the publication: withdraw unclaimed items, wait for running ones.
+    __device__ void spec_finish(const KParams& P) {
+        if (spec_pending) {
+            spec_pending = false;
+            if (lane_id() == 0) atomicSub(P.pending, 1u);
+        }
+        if (spec_slot < 0) return;
+        uint32_t* state = spec_state();
+        const uint32_t E = spec_epoch << 2;
+        if (lane_id() == 0) agent_store(&P.board[spec_slot], 1u);
+        group_sync();
+        for (uint32_t j = (uint32_t)lane_id(); j < spec_count; j += kWave) {
+            agent_cas(&state[j], E | 0u, E | 3u);
+            while (agent_load(&state[j]) == (E | 1u)) __builtin_amdgcn_s_sleep(2);
+        }
+        group_sync();
+        if (lane_id() == 0) agent_store(&P.board[spec_slot], 0u);
+        group_sync();
+        spec_slot = -1;
+    }
+
+    __device__ bool other_path_node(const KParams& P, uint32_t graph) {
+        // other_path_node (reference src/sequential.cpp:304-392)
+        if (ncells < n && (eqlevfirst == level || comp_canon >= 0)) {
+            if (tc_level_reached < level) {
+                if (!select_target()) return false;
+            }
+            const uint32_t li = level - 1;
+            LevelRec r = lv[li];
+            if (r.cursor >= r.tc_len) {
+                backtrack(level - 1);
+                return true;
+            }
+            if (num_aut > 0 && num_aut > r.last_num_aut) {
+                compute_orbits(li);
+                lv[li].last_num_aut = num_aut;
+            }
+            // next member that is its own orbit representative
+            const int l = lane_id();
+            uint32_t cur = r.cursor;
+            while (cur < r.tc_len) {
+                const uint32_t i = cur + (uint32_t)l;
+                const bool ok = i < r.tc_len && arena_rep[r.arena_off + i] == arena_mem[r.arena_off + i];
+                const uint64_t m = ballot(ok);
+                if (m) {
+                    cur += (uint32_t)__builtin_ctzll(m);
+                    break;
+                }
+                cur += kWave;
+            }
+            if (cur >= r.tc_len) {
+                lv[li].cursor = r.tc_len;
+                backtrack(level - 1);
+                return true;
+            }
+            r.cursor = cur;
+            if (level == 1 && spec_pending) spec_publish(P, r, graph);
+            const uint32_t v = arena_mem[r.arena_off + cur];
+            lv[li].cursor = cur + 1;
+            if (level == 1 && spec_take(v)) {
+                ++nodes;
+                seq[seq_size] = (ix_t)v;
+                ++seq_size;
+                ++level;
+                group_sync();
+                snapshot(level);
+            } else {
+                descend(v, r.tc_first);
+            }
+        }
+        classify();
+        return true;
+    }
+
+    __device__ void run(const KParams& P, uint32_t graph, int64_t vbase) {
+        nodes = 0;
+        steps = 0;
+        csr_bytes = 0;
+        status = 0;
+        num_aut = 0;
+        level = 0;
+        seq_size = 0;
+        tc_level_reached = 0;
+        eqlevfirst = 0;
+        eqlevcanon = 0;
+        comp_canon = 0;
+        canon_level = 0;
+        code = 0;
+        first_seq_size = best_seq_size = 0;
+        first_discovered = false;
+        budget = step_budget;
+        spec_slot = -1;
+        spec_maxn = P.maxn;
+        spec_pending = P.spec_min_n > 0 && n >= P.spec_min_n;
+        if (spec_pending && lane_id() == 0) atomicAdd(P.pending, 1u);
+        init_partition(P.init_order, P.color, vbase);
+        // search_tree_traversal (reference src/sequential.cpp:626-648)
+        code = refine(-1);
+        level = 1;
+        snapshot(1);
+        firstcode[0] = code;
+        ++nodes;
+        group_sync();
+        while (level >= 1 && status == 0) {
+            if (budget == 0) {
+                status = GCL_E_INTERNAL;
+                break;
+            }
+            --budget;
+            const bool ok = first_discovered ? other_path_node(P, graph) : first_path_node();
+            if (!ok) break;
+        }
+        spec_finish(P);
+    }
+
+    // Refines one published root child of another worker's graph.
+    __device__ void help_item(const KParams& P, uint8_t* rec, uint32_t j, uint32_t E) {
+        const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
+        const SpecHeader* hd = (const SpecHeader*)(rec + L.header);
+        const uint32_t graph = hd->graph;
+        const uint32_t tcf = hd->tc_first;
+        const int64_t vbase = P.vtx_off[graph];
+        n = (uint32_t)(P.vtx_off[graph + 1] - vbase);
+        nw = (n + 63) / 64;
+        rowp = P.row_ptr + vbase;
+        col = P.col + rowp[0];
+        H.bind(rowp, n, nullptr, P.maxn);
+        const ix_t* re = (const ix_t*)(rec + L.root_elem);
+        const uint64_t* rl = (const uint64_t*)(rec + L.root_lasts);
+        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) {
+            H.elem(p) = re[p];
+            H.cnt(p) = 0;
+            H.hits(p) = 0;
+            H.c1(p) = 0;
+            H.c2(p) = 0;
+            H.sa(p) = 0;
+        }
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) {
+            H.lasts()[w] = rl[w];
+            H.active()[w] = 0;
+            H.vmask()[w] = 0;
+        }
+        nactive = 0;
+        H.sync();
+        positions_from_elem();
+        rebuild_from_lasts();
+        const uint32_t v = ((const ix_t*)(rec + L.vert))[j];
+        const uint64_t s0 = steps;
+        budget = step_budget;
+        status = 0;
+        individualize_and_refine(v, tcf);
+        const uint64_t n1 = (uint64_t)P.maxn + 2, nw1 = (uint64_t)P.maxn / 64 + 2;
+        ix_t* oe = (ix_t*)(rec + L.elem) + (uint64_t)j * n1;
+        uint64_t* ol = (uint64_t*)(rec + L.lasts) + (uint64_t)j * nw1;
+        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) oe[p] = H.elem(p);
+        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) ol[w] = H.lasts()[w];
+        if (lane_id() == 0) {
+            ((uint32_t*)(rec + L.code))[j] = (uint32_t)code;
+            ((uint32_t*)(rec + L.ncells))[j] = ncells;
+            ((uint64_t*)(rec + L.steps))[j] = steps - s0;
+        }
+        __threadfence();
+        group_sync();
+        if (lane_id() == 0) agent_store(&((uint32_t*)(rec + L.state))[j], E | 2u);
+        group_sync();
+    }
+};
+
+// Tries to claim and refine one published root child.  Returns 1 when it did
+// some work, 0 when there was nothing to take now, -1 when helping is over.
+template <class Hot>
+__device__ int help_once(Worker<Hot>& W, const KParams& P) {
+    const int l = lane_id();
+    const uint32_t b = l < kBoard ? agent_load(&P.board[l]) : 0u;
+    const uint64_t pub = ballot(b == 2u);
+    if (!pub) {
+        const uint32_t pend = agent_load(P.pending);
+        return pend == 0 ? -1 : 0;
+    }
+    // start at a slot that depends on the worker, to spread helpers
+    const int rot = (int)(blockIdx.x % kBoard);
+    const uint64_t rotated = (pub >> rot) | (rot ? (pub << (kBoard - rot)) : 0ull);
+    const int slot = (__builtin_ctzll(rotated) + rot) % kBoard;
+    uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
+    const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
+    SpecHeader* hd = (SpecHeader*)(rec + L.header);
+    uint32_t claim = 0xFFFFFFFFu, E = 0;
+    if (l == 0) {
+        E = agent_load(&hd->epoch) << 2;
+        const uint32_t idx = atomicAdd(&hd->next, 1u);
+        if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], E | 0u, E | 1u)) claim = idx;
+    }
+    claim = __shfl(claim, 0, kWave);
+    E = __shfl(E, 0, kWave);
+    if (claim == 0xFFFFFFFFu) return 0;
+    W.help_item(P, rec, claim, E);
+    return 1;
+}
+
+template <class Hot>
+__device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, const Layout& L) {
+    W.snap = (uint64_t*)(base + L.snap);
+    W.seq = (ix_t*)(base + L.seq);
+    W.first_seq = (ix_t*)(base + L.first_seq);
+    W.best_seq = (ix_t*)(base + L.best_seq);
+    W.lv = (LevelRec*)(base + L.lv);
+    W.arena_mem = (ix_t*)(base + L.arena_mem);
+    W.arena_rep = (ix_t*)(base + L.arena_rep);
+    W.firstcode = (int32_t*)(base + L.firstcode);
+    W.canon_code = (int32_t*)(base + L.canon_code);
+    W.first_label = (ix_t*)(base + L.first_label);
+    W.best_label = (ix_t*)(base + L.best_label);
+    W.first_keys = (uint32_t*)(base + L.first_keys);
+    W.best_keys = (uint32_t*)(base + L.best_keys);
+    W.first_row = (uint32_t*)(base + L.first_row);
+    W.best_row = (uint32_t*)(base + L.best_row);
+    W.subgroup = (uint32_t*)(base + L.subgroup);
+    W.auts = (ix_t*)(base + L.auts);
+    W.arena_cap = P.arena_cap;
+    W.ring = P.ring;
+    W.levels = L.levels;
+    W.step_budget = P.step_budget;
+    W.steps = 0;
+}
+
+template <class Hot>
+__device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
+    const int l = lane_id();
+    bool graphs_left = true;
+    while (true) {
+        if (graphs_left) {
+            uint32_t item = 0;
+            if (l == 0) item = atomicAdd(P.next_item, 1u);
+            item = __shfl(item, 0, kWave);
+            if (item < (uint32_t)P.num_items) {
+                const int32_t g = P.items[item];
+                const int64_t vbase = P.vtx_off[g];
+                W.n = (uint32_t)(P.vtx_off[g + 1] - vbase);
+                W.nw = (W.n + 63) / 64;
+                W.rowp = P.row_ptr + vbase;
+                W.col = P.col + W.rowp[0];
+                W.nnz = (uint64_t)(W.rowp[W.n] - W.rowp[0]);
+                W.H.bind(W.rowp, W.n, nullptr, P.maxn);
+                W.H.sync();
+                const uint64_t t0 = wall_clock64();
+                W.run(P, (uint32_t)g, vbase);
+                const uint64_t t1 = wall_clock64();
+                for (uint32_t i = (uint32_t)l; i < W.n; i += kWave) P.out_label[vbase + i] = (int32_t)W.best_label[i];
+                if (l == 0) {
+                    DeviceGraphResult res;
+                    res.nodes = W.nodes;
+                    res.automorphisms = (uint64_t)W.num_aut;
+                    res.splitter_steps = W.steps;
+                    res.csr_bytes = W.csr_bytes;
+                    res.device_ns = t1 - t0;
+                    res.status = W.status;
+                    res.pad = 0;
+                    P.out[g] = res;
+                }
+                group_sync();
+                continue;
+            }
+            graphs_left = false;
+        }
+        if (P.spec_min_n == 0) break;
+        const int h = help_once(W, P);
+        if (h < 0) break;
+        if (h == 0) __builtin_amdgcn_s_sleep(20);
+    }
+}
+
+// The worker object lives in LDS: its uniform fields are reloaded where used
+// instead of occupying registers for the whole persistent loop.
+template <int MAXN>
+__global__ __launch_bounds__(kWave) void search_kernel_lds(KParams P) {
+    __shared__ Worker<HotL<MAXN>> W;
+    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false);
+    bind_scratch(W, P, base, L);
+    worker_loop(W, P);
+}
+
+__global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
+    __shared__ Worker<HotG> W;
+    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true);
+    bind_scratch(W, P, base, L);
+    W.H.h16 = (ix_t*)(base + L.hot16);
+    W.H.h32 = (uint32_t*)(base + L.hot32);
+    W.H.n1 = (uint64_t)P.maxn + 2;
+    worker_loop(W, P);
+}
+
+}  // namespace dev
+}  // namespace gcl

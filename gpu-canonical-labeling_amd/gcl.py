@@ -90,6 +90,7 @@ _SIGS = {
     "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_device_batch_destroy": (None, [C.c_void_p]),
+    "gcl_device_batch_graph_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_gen_mix_batch": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "gcl_owned_batch_free": (None, [C.c_void_p]),
 }
@@ -413,6 +414,13 @@ class DeviceBatch:
         _check(lib().gcl_device_batch_results(self.h, _ptr(labels), _ptr(nodes), C.byref(st)))
         return BatchResult(labels[:int(b.vtx_off[-1])], nodes[:b.num_graphs], st.nodes, st.automorphisms,
                            st.splitter_steps, st.csr_bytes, 0.0, 0.0, st.workers)
+
+    def graph_stats(self):
+        """Per-graph (nodes, automorphisms, splitter_steps, device_ns) of the last run."""
+        G = max(self.batch.num_graphs, 1)
+        a = [np.zeros(G, np.uint64) for _ in range(4)]
+        _check(lib().gcl_device_batch_graph_stats(self.h, *[_ptr(x) for x in a]))
+        return tuple(x[:self.batch.num_graphs] for x in a)
 
     def close(self):
         if self.h:

@@ -169,6 +169,11 @@ gcl_status gcl_device_batch_create(const gcl_batch* b, uint32_t max_aut, gcl_dev
 gcl_status gcl_device_batch_run(gcl_device_batch* db, int32_t use_graph, gcl_batch_stats* stats);
 gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, uint64_t* out_nodes,
                                     gcl_batch_stats* stats);
+/* Per-graph records of the last run (G entries each; any pointer may be
+ * NULL): search-tree nodes, automorphisms, splitter steps and the device wall
+ * time of the graph's search in nanoseconds. */
+gcl_status gcl_device_batch_graph_stats(gcl_device_batch* db, uint64_t* nodes, uint64_t* automorphisms,
+                                        uint64_t* splitter_steps, uint64_t* device_ns);
 void gcl_device_batch_destroy(gcl_device_batch* db);
 
 /* Canonical keys of one graph from a labeling: out_keys[nnz] = sorted
