@@ -36,5 +36,6 @@ int32_t device_batch_size(const DeviceBatch* db);
 int32_t device_count();
 uint64_t step_budget();
 uint32_t spec_min_n();
+uint64_t time_limit_ticks();
 
 }  // namespace gcl

@@ -91,6 +91,17 @@ uint64_t step_budget() {
     return 1ull << 40;
 }
 
+// Wall-clock limit of a launch in device ticks (GCL_TIME_LIMIT_MS; default
+// none): past it every worker stops with GCL_E_INTERNAL instead of running on.
+uint64_t time_limit_ticks() {
+    const char* e = getenv("GCL_TIME_LIMIT_MS");
+    if (!e || !*e) return ~0ull;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    return (uint64_t)strtoull(e, nullptr, 10) * (uint64_t)rate_khz;
+}
+
 // Graphs with at least this many vertices publish their root children for
 // idle workers (0 disables); GCL_SPEC_MIN_N overrides the default.
 uint32_t spec_min_n() {
@@ -115,6 +126,7 @@ struct DeviceBatch {
         size_t item_off = 0;
         KParams P;
         DevBuf scratch, spec, board;
+        uint64_t spec_work = 0;  // root children of the class's large graphs
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
     };
@@ -157,6 +169,7 @@ struct DeviceBatch {
         std::vector<std::vector<int32_t>> items_h(kClasses + 1);
         std::vector<uint32_t> cls_maxn(kClasses + 1, 0);
         std::vector<uint64_t> cls_maxnnz(kClasses + 1, 0);
+        std::vector<uint64_t> cls_spec_work(kClasses + 1, 0);
         for (int32_t g = 0; g < G; ++g) {
             const int64_t n = b.vtx_off[g + 1] - b.vtx_off[g];
             if (n > kMaxVertices)
@@ -171,6 +184,7 @@ struct DeviceBatch {
             cls_maxn[cls] = std::max(cls_maxn[cls], (uint32_t)n);
             const uint64_t nnz = (uint64_t)(b.row_ptr[b.vtx_off[g + 1]] - b.row_ptr[b.vtx_off[g]]);
             cls_maxnnz[cls] = std::max(cls_maxnnz[cls], nnz);
+            if ((uint32_t)n >= spec_min_n()) cls_spec_work[cls] += (uint64_t)n;
         }
         for (auto& v : items_h)
             std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
@@ -242,8 +256,8 @@ struct DeviceBatch {
             const int64_t mem_workers = (int64_t)((double)free_b * 0.5 / (double)L.total);
             // with speculation every resident worker runs: the ones without a
             // graph refine published root children of the others
-            int64_t grid = std::min<int64_t>({spec ? resident : (int64_t)cnt, resident,
-                                              std::max<int64_t>(mem_workers, 1)});
+            const int64_t want = spec ? (int64_t)(cnt + cls_spec_work[cls]) : (int64_t)cnt;
+            int64_t grid = std::min<int64_t>({want, resident, std::max<int64_t>(mem_workers, 1)});
             C.grid = std::max<int64_t>(grid, 1);
             uint32_t spec_slots = 0;
             uint64_t spec_stride = 0;
@@ -256,8 +270,8 @@ struct DeviceBatch {
                 spec_stride = make_spec_layout(maxn, spec_slots).total;
                 GCL_HIP(hipMalloc(&C.spec.p, (size_t)(spec_stride * kBoard)));
                 GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(spec_stride * kBoard)));
-                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned int) * kBoard));
-                GCL_HIP(hipMemset(C.board.p, 0, sizeof(unsigned int) * kBoard));
+                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned int) * kBoard + sizeof(unsigned long long)));
+                GCL_HIP(hipMemset(C.board.p, 0, sizeof(unsigned int) * kBoard + sizeof(unsigned long long)));
             }
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
             GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
@@ -286,7 +300,9 @@ struct DeviceBatch {
             P.spec_stride = spec_stride;
             P.spec_slots = spec_slots;
             P.spec_min_n = spec ? spec_min : 0u;
+            P.time_limit = time_limit_ticks();
             P.board = (unsigned int*)C.board.p;
+            P.board_mask = spec ? (unsigned long long*)((unsigned int*)C.board.p + kBoard) : nullptr;
             P.pending = (unsigned int*)ctr.p + 32 + cls;
             workers += (int32_t)C.grid;
             item_off += cnt;

@@ -148,7 +148,9 @@ struct KParams {
     uint32_t spec_slots;
     uint32_t spec_min_n;        // 0 disables speculation
     unsigned int* board;        // kBoard words: 0 free, 1 owned, 2 published
+    unsigned long long* board_mask;  // bit b set while slot b is published
     unsigned int* pending;      // running graphs that may still publish
+    uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
 };
 
 // ------------------------------------------------------------ wave helpers
@@ -306,7 +308,7 @@ struct Worker {
     uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
     uint64_t arena_cap;
     uint32_t ring, levels;
-    uint64_t step_budget;
+    uint64_t step_budget, deadline;
     // search state (reference src/sequential.h:24-58)
     uint32_t level, seq_size, tc_level_reached, eqlevfirst, first_seq_size, best_seq_size;
     int eqlevcanon, comp_canon, canon_level, code;
@@ -1307,6 +1309,8 @@ struct Worker {
         if (lane_id() == 0) {
             agent_store(&hd->epoch, epoch);
             agent_store(&P.board[slot], 2u);
+            __threadfence();
+            atomicOr(P.board_mask, (unsigned long long)(1ull << slot));
         }
         group_sync();
     }
@@ -1326,13 +1330,15 @@ struct Worker {
         if (spec_cursor >= spec_count || (uint32_t)vert[spec_cursor] != v) return false;
         const uint32_t j = spec_cursor++;
         uint32_t won = 0;
-        if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 1u) ? 1u : 0u;
+        if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 3u) ? 1u : 0u;
         won = __shfl(won, 0, kWave);
-        if (won) return false;  // refine it here (the item is withdrawn)
+        if (won) return false;  // withdrawn: refine it here
         // a helper has it: wait until done, then copy its partition
-        for (uint64_t spin = 0;; ++spin) {
-            const uint32_t st = agent_load(&state[j]);
-            if (st == (E | 2u)) break;
+        while (agent_load(&state[j]) != (E | 2u)) {
+            if (wall_clock64() > deadline) {
+                status = GCL_E_INTERNAL;  // a helper did not finish in time
+                return false;
+            }
             __builtin_amdgcn_s_sleep(2);
         }
         const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
@@ -1361,9 +1367,11 @@ struct Worker {
         const uint32_t E = spec_epoch << 2;
         if (lane_id() == 0) agent_store(&P.board[spec_slot], 1u);
         group_sync();
+        if (lane_id() == 0)
+            atomicAnd(P.board_mask, ~(unsigned long long)(1ull << spec_slot));
         for (uint32_t j = (uint32_t)lane_id(); j < spec_count; j += kWave) {
             agent_cas(&state[j], E | 0u, E | 3u);
-            while (agent_load(&state[j]) == (E | 1u)) __builtin_amdgcn_s_sleep(2);
+            while (agent_load(&state[j]) == (E | 1u) && wall_clock64() <= deadline) __builtin_amdgcn_s_sleep(2);
         }
         group_sync();
         if (lane_id() == 0) agent_store(&P.board[spec_slot], 0u);
@@ -1454,7 +1462,7 @@ struct Worker {
         ++nodes;
         group_sync();
         while (level >= 1 && status == 0) {
-            if (budget == 0) {
+            if (budget == 0 || wall_clock64() > deadline) {
                 status = GCL_E_INTERNAL;
                 break;
             }
@@ -1523,29 +1531,42 @@ struct Worker {
 template <class Hot>
 __device__ int help_once(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
-    const uint32_t b = l < kBoard ? agent_load(&P.board[l]) : 0u;
-    const uint64_t pub = ballot(b == 2u);
-    if (!pub) {
-        const uint32_t pend = agent_load(P.pending);
-        return pend == 0 ? -1 : 0;
+    uint64_t pub = 0;
+    uint32_t pend = 0;
+    if (l == 0) {
+        pub = __hip_atomic_load(P.board_mask, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!pub) pend = agent_load(P.pending);
     }
-    // start at a slot that depends on the worker, to spread helpers
+    pub = ((uint64_t)__shfl((uint32_t)(pub >> 32), 0, kWave) << 32) | __shfl((uint32_t)pub, 0, kWave);
+    pend = __shfl(pend, 0, kWave);
+    if (!pub) return pend == 0 ? -1 : 0;
+    const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
+    // visit published slots starting at one that depends on the worker
     const int rot = (int)(blockIdx.x % kBoard);
     const uint64_t rotated = (pub >> rot) | (rot ? (pub << (kBoard - rot)) : 0ull);
-    const int slot = (__builtin_ctzll(rotated) + rot) % kBoard;
-    uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
-    const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
-    SpecHeader* hd = (SpecHeader*)(rec + L.header);
-    uint32_t claim = 0xFFFFFFFFu, E = 0;
+    uint32_t claim = 0xFFFFFFFFu, E = 0, slot_sel = 0;
     if (l == 0) {
-        E = agent_load(&hd->epoch) << 2;
-        const uint32_t idx = atomicAdd(&hd->next, 1u);
-        if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], E | 0u, E | 1u)) claim = idx;
+        for (uint64_t m = rotated; m; m &= m - 1) {
+            const int slot = (__builtin_ctzll(m) + rot) % kBoard;
+            uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
+            SpecHeader* hd = (SpecHeader*)(rec + L.header);
+            const uint32_t e = agent_load(&hd->epoch) << 2;
+            const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
+            const uint32_t idx = atomicAdd(&hd->next, 1u);
+            if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
+                claim = idx;
+                E = e;
+                slot_sel = (uint32_t)slot;
+                break;
+            }
+        }
     }
     claim = __shfl(claim, 0, kWave);
-    E = __shfl(E, 0, kWave);
     if (claim == 0xFFFFFFFFu) return 0;
-    W.help_item(P, rec, claim, E);
+    E = __shfl(E, 0, kWave);
+    slot_sel = __shfl(slot_sel, 0, kWave);
+    W.help_item(P, P.spec + (uint64_t)slot_sel * P.spec_stride, claim, E);
     return 1;
 }
 
@@ -1572,6 +1593,8 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.ring = P.ring;
     W.levels = L.levels;
     W.step_budget = P.step_budget;
+    const uint64_t now = wall_clock64();
+    W.deadline = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
     W.steps = 0;
 }
 
@@ -1579,6 +1602,7 @@ template <class Hot>
 __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
     bool graphs_left = true;
+    uint32_t idle = 1;
     while (true) {
         if (graphs_left) {
             uint32_t item = 0;
@@ -1614,10 +1638,16 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             }
             graphs_left = false;
         }
-        if (P.spec_min_n == 0) break;
+        if (P.spec_min_n == 0 || wall_clock64() > W.deadline) break;
         const int h = help_once(W, P);
         if (h < 0) break;
-        if (h == 0) __builtin_amdgcn_s_sleep(20);
+        if (h == 0) {
+            // idle: back off (up to ~100 us) to keep the polling traffic low
+            for (uint32_t i = 0; i < idle; ++i) __builtin_amdgcn_s_sleep(127);
+            idle = idle < 32 ? idle * 2 : 32;
+        } else {
+            idle = 1;
+        }
     }
 }
 

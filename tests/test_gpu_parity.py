@@ -121,3 +121,27 @@ def test_gpu_nauty_parallel_dense():
     o = oracle.canon_graph(g)
     assert np.array_equal(dense.reshape(g.n, g.n), o["dense"])
     assert nodes == o["nodes"]
+
+
+def test_gpu_speculation_keeps_traversal_exact(monkeypatch):
+    # root children refined by idle workers must not change the traversal:
+    # same canonical form, node and automorphism counts as the oracle, with
+    # speculation forced on for small graphs and switched off
+    rs = np.random.RandomState(7)
+    er393 = None
+    for s in range(12):
+        n, d = int(rs.randint(10, 400)), float(rs.uniform(1.0, 10.0))
+        g = gcl.erdos_renyi_mean_degree(n, d, s)
+        if n == 393:
+            er393 = g
+    graphs = [er393, gcl.random_regular(300, 3, 21), gcl.torus(12, 12), gcl.cfi(20, True, 5),
+              gcl.erdos_renyi_mean_degree(150, 2.0, 3)]
+    for spec in ("64", "0"):
+        monkeypatch.setenv("GCL_SPEC_MIN_N", spec)
+        for g in graphs:
+            for max_aut in (10000, 1):
+                _log(f"speculation min_n={spec} n={g.n} m={g.m} max_aut={max_aut}")
+                o = oracle.canon_graph(g, max_aut=max_aut)
+                r = gcl.canon_label(g, max_aut=max_aut, engine="gpu")
+                assert np.array_equal(r.keys, o["keys"]), g.n
+                assert (r.nodes, r.automorphisms) == (o["nodes"], o["automorphisms"]), (g.n, spec, max_aut)
