@@ -75,9 +75,9 @@ const void* class_kernel(int cls) {
 
 // Zeroes the per-class work counters at the start of a run (a kernel rather
 // than a memset, so that the run can be captured in a graph).
-__global__ void reset_counters(unsigned int* ctr, int count) {
+__global__ void reset_counters(unsigned int* ctr, const unsigned int* init, int count) {
     int i = (int)threadIdx.x;
-    if (i < count) ctr[i] = 0u;
+    if (i < count) ctr[i] = init[i];
 }
 
 }  // namespace
@@ -127,13 +127,14 @@ struct DeviceBatch {
         KParams P;
         DevBuf scratch, spec, board;
         uint64_t spec_work = 0;  // root children of the class's large graphs
+        uint32_t spec_big = 0;   // graphs of the class that may publish
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
     };
     int32_t G = 0;
     int64_t total_n = 0, total_nnz = 0;
     uint32_t max_aut = 1, ring = 1;
-    DevBuf vtx, row, col, color, init, label, out, items, ctr;
+    DevBuf vtx, row, col, color, init, label, out, items, ctr, ctr_init;
     std::vector<Cls> classes;
     hipStream_t main = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, start = nullptr;
@@ -170,6 +171,7 @@ struct DeviceBatch {
         std::vector<uint32_t> cls_maxn(kClasses + 1, 0);
         std::vector<uint64_t> cls_maxnnz(kClasses + 1, 0);
         std::vector<uint64_t> cls_spec_work(kClasses + 1, 0);
+        std::vector<uint32_t> cls_spec_big(kClasses + 1, 0);
         for (int32_t g = 0; g < G; ++g) {
             const int64_t n = b.vtx_off[g + 1] - b.vtx_off[g];
             if (n > kMaxVertices)
@@ -184,7 +186,10 @@ struct DeviceBatch {
             cls_maxn[cls] = std::max(cls_maxn[cls], (uint32_t)n);
             const uint64_t nnz = (uint64_t)(b.row_ptr[b.vtx_off[g + 1]] - b.row_ptr[b.vtx_off[g]]);
             cls_maxnnz[cls] = std::max(cls_maxnnz[cls], nnz);
-            if ((uint32_t)n >= spec_min_n()) cls_spec_work[cls] += (uint64_t)n;
+            if ((uint32_t)n >= spec_min_n()) {
+                cls_spec_work[cls] += (uint64_t)n;
+                cls_spec_big[cls] += 1;
+            }
         }
         for (auto& v : items_h)
             std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
@@ -210,6 +215,7 @@ struct DeviceBatch {
         GCL_HIP(hipMalloc(&out.p, sizeof(DeviceGraphResult) * (size_t)std::max(G, 1)));
         GCL_HIP(hipMalloc(&items.p, sizeof(int32_t) * (size_t)std::max(G, 1)));
         GCL_HIP(hipMalloc(&ctr.p, sizeof(unsigned int) * 64));
+        GCL_HIP(hipMalloc(&ctr_init.p, sizeof(unsigned int) * 64));
         GCL_HIP(hipMemcpy(vtx.p, b.vtx_off, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice));
         GCL_HIP(hipMemcpy(row.p, b.row_ptr, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice));
         if (total_nnz > 0)
@@ -259,19 +265,26 @@ struct DeviceBatch {
             const int64_t want = spec ? (int64_t)(cnt + cls_spec_work[cls]) : (int64_t)cnt;
             int64_t grid = std::min<int64_t>({want, resident, std::max<int64_t>(mem_workers, 1)});
             C.grid = std::max<int64_t>(grid, 1);
-            uint32_t spec_slots = 0;
+            uint32_t spec_slots = 0, spec_records = 0;
             uint64_t spec_stride = 0;
             if (spec) {
-                // one record per board slot; a record holds every root child
-                // of a graph unless that exceeds the memory budget
+                // one record per graph that can publish concurrently; a record
+                // holds every root child unless that exceeds the memory budget
+                spec_records = (uint32_t)std::min<uint64_t>({(uint64_t)cls_spec_big[cls], (uint64_t)C.grid,
+                                                             (uint64_t)kMaxRecords});
                 const uint64_t per_slot = make_spec_layout(maxn, 2).total - make_spec_layout(maxn, 1).total + 64;
-                const uint64_t budget = 4ull << 30;
-                spec_slots = (uint32_t)std::min<uint64_t>(maxn, std::max<uint64_t>(64, budget / kBoard / per_slot));
+                const uint64_t budget = 8ull << 30;
+                spec_slots = (uint32_t)std::min<uint64_t>(
+                    maxn, std::max<uint64_t>(64, budget / std::max<uint32_t>(spec_records, 1) / per_slot));
                 spec_stride = make_spec_layout(maxn, spec_slots).total;
-                GCL_HIP(hipMalloc(&C.spec.p, (size_t)(spec_stride * kBoard)));
-                GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(spec_stride * kBoard)));
-                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned int) * kBoard + sizeof(unsigned long long)));
-                GCL_HIP(hipMemset(C.board.p, 0, sizeof(unsigned int) * kBoard + sizeof(unsigned long long)));
+                GCL_HIP(hipMalloc(&C.spec.p, (size_t)(spec_stride * spec_records)));
+                GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(spec_stride * spec_records)));
+                const uint32_t nwr = (spec_records + 63) / 64;
+                std::vector<unsigned long long> masks(2 * (size_t)nwr, 0ull);
+                for (uint32_t r = 0; r < spec_records; ++r) masks[r / 64] |= 1ull << (r % 64);  // all free
+                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned long long) * masks.size()));
+                GCL_HIP(hipMemcpy(C.board.p, masks.data(), sizeof(unsigned long long) * masks.size(),
+                                  hipMemcpyHostToDevice));
             }
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
             GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
@@ -301,19 +314,28 @@ struct DeviceBatch {
             P.spec_slots = spec_slots;
             P.spec_min_n = spec ? spec_min : 0u;
             P.time_limit = time_limit_ticks();
-            P.board = (unsigned int*)C.board.p;
-            P.board_mask = spec ? (unsigned long long*)((unsigned int*)C.board.p + kBoard) : nullptr;
+            P.spec_records = spec_records;
+            P.slot_free = spec ? (unsigned long long*)C.board.p : nullptr;
+            P.board_mask = spec ? (unsigned long long*)C.board.p + (spec_records + 63) / 64 : nullptr;
             P.pending = (unsigned int*)ctr.p + 32 + cls;
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));
         }
+        // per-class work counters start at 0; the pending counters start at
+        // the number of graphs that may publish root children, so helpers
+        // never leave before those graphs have published or finished
+        std::vector<unsigned int> init_ctr(64, 0u);
+        for (auto& c : classes)
+            if (c.P.spec_min_n) init_ctr[32 + c.cls] = cls_spec_big[c.cls];
+        GCL_HIP(hipMemcpy(ctr_init.p, init_ctr.data(), sizeof(unsigned int) * 64, hipMemcpyHostToDevice));
         return GCL_OK;
     }
 
     // reset counters, fork one stream per size class, join
     hipError_t enqueue() {
-        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p, 64);
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p,
+                           (const unsigned int*)ctr_init.p, 64);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if ((e = hipEventRecord(start, main)) != hipSuccess) return e;

@@ -35,7 +35,7 @@ namespace dev {
 
 typedef uint16_t ix_t;
 constexpr int kWave = 64;
-constexpr int kBoard = 64;          // concurrent speculating graphs per launch
+constexpr uint32_t kMaxRecords = 4096;  // speculation records per launch (64 mask words)
 constexpr uint32_t kNoKey = 0u;
 
 struct LevelRec {
@@ -143,12 +143,13 @@ struct KParams {
     uint32_t levels;
     uint64_t step_budget;
     // root speculation
-    uint8_t* spec;              // kBoard records
+    uint8_t* spec;              // spec_records records
     uint64_t spec_stride;
-    uint32_t spec_slots;
+    uint32_t spec_slots;        // root children per record
+    uint32_t spec_records;      // <= kMaxRecords
     uint32_t spec_min_n;        // 0 disables speculation
-    unsigned int* board;        // kBoard words: 0 free, 1 owned, 2 published
-    unsigned long long* board_mask;  // bit b set while slot b is published
+    unsigned long long* slot_free;   // bit r set while record r is free
+    unsigned long long* board_mask;  // bit r set while record r is published
     unsigned int* pending;      // running graphs that may still publish
     uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
 };
@@ -362,8 +363,8 @@ struct Worker {
             const uint64_t b = ballot(bits != 0ull);
             if (b) {
                 const int fl = __builtin_ctzll(b);
-                const uint32_t lo = __shfl((uint32_t)bits, fl, kWave);
-                const uint32_t hi = __shfl((uint32_t)(bits >> 32), fl, kWave);
+                const uint32_t lo = readlane((uint32_t)bits, fl);
+                const uint32_t hi = readlane((uint32_t)(bits >> 32), fl);
                 const uint64_t word = ((uint64_t)hi << 32) | lo;
                 return (int32_t)((base + (uint32_t)fl) * 64u + (uint32_t)__builtin_ctzll(word));
             }
@@ -558,10 +559,8 @@ struct Worker {
         }
         ncells += nsplit;
         nactive += nsplit;
-        // hint: the last split in position order that sets one
-        const uint32_t hk = wave_max(hkey);
-        if (hk != kNoKey) hint = (int)(hk & 0xFFFFu);
-        // boundary hash chain in position order (399-409): x = f+k-1 grows with f
+        // boundary hash chain in position order (399-409): x = f+k-1 grows
+        // with f; the hint is the candidate of the last split that has one
         const uint32_t x = f + k - 1;
         uint32_t rank = 0;
         for (uint64_t m = om; m; m &= m - 1) {
@@ -569,18 +568,85 @@ struct Worker {
             if (own && xj < x) ++rank;
         }
         for (uint32_t t = 0; t < nsplit; ++t) {
-            const uint64_t b = ballot(own && rank == t);
-            longcode = GCL_MASH(longcode, readlane(x, __builtin_ctzll(b)));
+            const int j = __builtin_ctzll(ballot(own && rank == t));
+            longcode = GCL_MASH(longcode, readlane(x, j));
+            const uint32_t hj = readlane(hkey, j);
+            if (hj != kNoKey) hint = (int)(hj & 0xFFFFu);
         }
         H.sync();
     }
 
+    // Regroups one touched cell [f, c] of at most kSmallCell members by its
+    // key, entirely in this lane (sorting network in registers).  Returns the
+    // number of fragments; records the cell's state for the hash pass.
+    static constexpr uint32_t kSmallCell = 8;
+    __device__ __forceinline__ uint32_t regroup_small(uint32_t f, uint32_t c, uint32_t s) {
+        const uint32_t L = c - f + 1;
+        uint32_t kk[kSmallCell];
+#pragma unroll
+        for (uint32_t t = 0; t < kSmallCell; ++t) {
+            if (t < L) {
+                const uint32_t v = H.elem(f + t);
+                const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                kk[t] = (kv << 16) | v;
+            } else {
+                kk[t] = 0xFFFFFFFFu;
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 2; k <= kSmallCell; k <<= 1) {
+#pragma unroll
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (uint32_t i = 0; i < kSmallCell; ++i) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & k) == 0;
+                        const uint32_t a = kk[i], b = kk[ixj];
+                        if ((a > b) == up) {
+                            kk[i] = b;
+                            kk[ixj] = a;
+                        }
+                    }
+                }
+            }
+        }
+        uint32_t kmin = kk[0] >> 16, kmax = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < kSmallCell; ++t)
+            if (t + 1 == L) kmax = kk[t] >> 16;
+        if (kmin == kmax) {
+            H.sa(f) = 1;
+            H.sv(f) = (ix_t)((s == 1) ? 1u - kmin : kmin);
+            return 1;
+        }
+        uint32_t nfrag = 0, end = 0;
+#pragma unroll
+        for (int t = (int)kSmallCell - 1; t >= 0; --t) {
+            if ((uint32_t)t < L) {
+                const uint32_t key = kk[t] >> 16, v = kk[t] & 0xFFFFu;
+                if ((uint32_t)t + 1 == L || (kk[t + 1 < (int)kSmallCell ? t + 1 : t] >> 16) != key) end = (uint32_t)t;
+                H.elem(f + (uint32_t)t) = (ix_t)v;
+                H.pos(v) = (ix_t)(f + (uint32_t)t);
+                H.cend(f + (uint32_t)t) = (ix_t)(f + end);
+                if (t == 0 || (kk[t > 0 ? t - 1 : 0] >> 16) != key) {
+                    H.cfirst(f + end) = (ix_t)(f + (uint32_t)t);
+                    if (f + end != c) set_bit_atomic(H.lasts(), f + end);
+                    ++nfrag;
+                }
+            }
+        }
+        H.sa(f) = 2;
+        H.sv(f) = (ix_t)c;
+        return nfrag;
+    }
+
     // General refinement step: splitter cell [p, e] of any size (reference
-    // src/partition_refinement.cpp:197-473): counts with LDS atomics, every
-    // touched cell regrouped by a counting sort on its key (count for
-    // |W| > 1; neighbours first for |W| == 1), then one pass over the
-    // pre-step non-singleton cells in position order for the hash chain and
-    // the active-set rules.
+    // src/partition_refinement.cpp:197-473): counts with LDS atomics; touched
+    // cells regrouped by key (count for |W| > 1; neighbours first for
+    // |W| == 1), small cells one per lane, large ones by the whole wave; then
+    // the hash chain and active-set rules over the pre-step non-singleton
+    // cells in position order.
     __device__ void general_step(uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         const uint32_t s = e - p + 1;
@@ -624,8 +690,31 @@ struct Worker {
         }
         H.sync();
         const uint32_t ntc = ctr[1];
-        for (uint32_t j = 0; j < ntc; ++j) {
-            const uint32_t c = H.tcl(j);
+        // small cells: one lane each; large ones listed after the touched list
+        uint32_t nbig = 0, nfrag_extra = 0;
+        for (uint32_t base = 0; base < ntc; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            bool big = false;
+            uint32_t c = 0;
+            if (i < ntc) {
+                c = H.tcl(i);
+                const uint32_t f = H.cfirst(c);
+                H.hits(c) = 0;
+                if (c - f + 1 <= kSmallCell) {
+                    nfrag_extra += regroup_small(f, c, s) - 1;
+                    H.tcl(i) = (ix_t)f;  // now the old first
+                } else {
+                    big = true;
+                }
+            }
+            const uint64_t bm = ballot(big);
+            if (big) H.tcl(ntc + nbig + lanes_below(bm)) = (ix_t)c;
+            nbig += (uint32_t)__popcll(bm);
+        }
+        ncells += wave_sum(nfrag_extra);
+        H.sync();
+        for (uint32_t j = 0; j < nbig; ++j) {
+            const uint32_t c = H.tcl(ntc + j);
             const uint32_t f = H.cfirst(c);
             const uint32_t L = c - f + 1;
             uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
@@ -640,12 +729,10 @@ struct Worker {
             }
             kmin = wave_min(kmin);
             kmax = wave_max(kmax);
-            if (l == 0) H.hits(c) = 0;
             if (kmin == kmax) {
                 if (l == 0) {
                     H.sa(f) = 1;
                     H.sv(f) = (ix_t)((s == 1) ? 1u - kmin : kmin);
-                    H.tcl(j) = (ix_t)f;
                 }
                 H.sync();
                 continue;
@@ -668,7 +755,7 @@ struct Worker {
                 const uint32_t sz = b < R ? H.c1(b) : 0u;
                 const uint32_t inc = wave_scan(sz);
                 if (b < R) H.c2(b) = run + inc - sz;
-                run += __shfl(inc, kWave - 1, kWave);
+                run += readlane(inc, kWave - 1);
                 nfrag += (uint32_t)__popcll(ballot(sz != 0));
             }
             H.sync();
@@ -708,42 +795,61 @@ struct Worker {
             if (l == 0) {
                 H.sa(f) = 2;
                 H.sv(f) = (ix_t)c;
-                H.tcl(j) = (ix_t)f;
             }
             H.sync();
         }
-        // hash chain / active rules over the pre-step non-singleton cells
-        const uint64_t* nsm = H.nsmask();
-        for (uint32_t wi = 0; wi < nw; ++wi) {
-            uint64_t bits = nsm[wi];
+        // compact list of the pre-step non-singleton cells (position order)
+        uint32_t nns = 0;
+        for (uint32_t base = 0; base < nw; base += kWave) {
+            const uint32_t w = base + (uint32_t)l;
+            uint64_t bits = w < nw ? H.nsmask()[w] : 0ull;
+            const uint32_t c0 = (uint32_t)__popcll(bits);
+            const uint32_t inc = wave_scan(c0);
+            uint32_t j = nns + inc - c0;
             while (bits) {
-                const uint32_t f = wi * 64u + (uint32_t)__builtin_ctzll(bits);
+                H.sb(j++) = (ix_t)(w * 64u + (uint32_t)__builtin_ctzll(bits));
                 bits &= bits - 1;
-                const uint32_t st = H.sa(f);
-                if (st == 0) {
-                    if (s > 1) longcode = GCL_MASH(longcode, f);
+            }
+            nns += readlane(inc, kWave - 1);
+        }
+        H.sync();
+        // hash chain / active rules in position order (reference 253-468)
+        for (uint32_t base = 0; base < nns; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            uint32_t f = 0, st = 0, x = 0;
+            if (i < nns) {
+                f = H.sb(i);
+                st = H.sa(f);
+                x = (st == 1) ? (uint32_t)H.sv(f) + f : f;
+            }
+            const uint64_t special = ballot(i < nns && st == 2);
+            const uint32_t here = min((uint32_t)kWave, nns - base);
+            if (s > 1 && special == 0) {
+                for (uint32_t t = 0; t < here; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
+                continue;
+            }
+            for (uint32_t t = 0; t < here; ++t) {
+                if (!((special >> t) & 1ull)) {
+                    if (s > 1) longcode = GCL_MASH(longcode, readlane(x, (int)t));
                     continue;
                 }
-                if (st == 1) {
-                    if (s > 1) longcode = GCL_MASH(longcode, (uint32_t)H.sv(f) + f);
-                    continue;
-                }
-                const uint32_t cold = H.sv(f);
+                const uint32_t fc = readlane(f, (int)t);
+                const uint32_t cold = H.sv(fc);
                 if (s == 1) {
-                    const uint32_t K = (uint32_t)H.cend(f) - f + 1;
-                    const uint32_t LL = cold - f + 1;
-                    longcode = GCL_MASH(longcode, f + K - 1);
-                    if (test_bit(H.active(), f) || K >= LL - K) {
-                        add_active(f + K);
-                        if (LL - K == 1) hint = (int)(f + K);
+                    const uint32_t K = (uint32_t)H.cend(fc) - fc + 1;
+                    const uint32_t LL = cold - fc + 1;
+                    longcode = GCL_MASH(longcode, fc + K - 1);
+                    if (test_bit(H.active(), fc) || K >= LL - K) {
+                        add_active(fc + K);
+                        if (LL - K == 1) hint = (int)(fc + K);
                     } else {
-                        add_active(f);
-                        if (K == 1) hint = (int)f;
+                        add_active(fc);
+                        if (K == 1) hint = (int)fc;
                     }
                     continue;
                 }
-                uint32_t max_size = 0, max_first = f;
-                for (uint32_t q = f; q <= cold;) {
+                uint32_t max_size = 0, max_first = fc;
+                for (uint32_t q = fc; q <= cold;) {
                     const uint32_t last = H.cend(q);
                     const uint32_t size = last - q + 1;
                     longcode = GCL_MASH(longcode, H.cnt(H.elem(q)) + q);
@@ -753,26 +859,28 @@ struct Worker {
                     }
                     q = last + 1;
                 }
-                const bool was_active = test_bit(H.active(), f);
-                for (uint32_t q = (uint32_t)H.cend(f) + 1; q <= cold;) {
+                const bool was_active = test_bit(H.active(), fc);
+                for (uint32_t q = (uint32_t)H.cend(fc) + 1; q <= cold;) {
                     const uint32_t last = H.cend(q);
                     add_active(q);
                     if (last == q) hint = (int)q;
                     q = last + 1;
                 }
                 if (!was_active) {
-                    add_active(f);
+                    add_active(fc);
                     remove_active(max_first);
                 }
             }
         }
         H.sync();
-        // cell masks of the new fragments; reset per-cell state
-        for (uint32_t base = 0; base < ntc; base += kWave) {
+        // cell masks of the new fragments; reset the per-cell states.  The
+        // states are keyed by old first positions: walk the ns list again.
+        for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            if (i < ntc) {
-                const uint32_t f = H.tcl(i);
-                if (H.sa(f) == 2) {
+            if (i < nns) {
+                const uint32_t f = H.sb(i);
+                const uint32_t st = H.sa(f);
+                if (st == 2) {
                     const uint32_t cold = H.sv(f);
                     for (uint32_t q = f; q <= cold;) {
                         const uint32_t last = H.cend(q);
@@ -781,7 +889,7 @@ struct Worker {
                         q = last + 1;
                     }
                 }
-                H.sa(f) = 0;
+                if (st) H.sa(f) = 0;
             }
         }
         H.sync();
@@ -932,23 +1040,26 @@ struct Worker {
         const LevelRec r = lv[li];
         const int64_t upto = min(num_aut, (int64_t)ring);
         const int64_t from = r.last_num_aut < 0 ? 0 : r.last_num_aut;
+        // ring slots whose automorphism fixes the sequence: one slot per lane
         uint32_t nsub = 0;
-        for (int64_t i = from; i < upto; ++i) {
-            const ix_t* a = auts + (uint64_t)i * n;
-            bool bad = false;
-            for (uint32_t base = 0; base < seq_size; base += kWave) {
-                const uint32_t j = base + (uint32_t)l;
-                const bool b = j < seq_size && a[seq[j]] != seq[j];
-                if (ballot(b)) {
-                    bad = true;
-                    break;
+        for (int64_t base = from; base < upto; base += kWave) {
+            const int64_t i = base + l;
+            bool ok = i < upto;
+            if (ok) {
+                const ix_t* a = auts + (uint64_t)i * n;
+                for (uint32_t j = 0; j < seq_size; ++j) {
+                    const uint32_t sj = seq[j];
+                    if (a[sj] != sj) {
+                        ok = false;
+                        break;
+                    }
                 }
             }
-            if (!bad) {
-                subgroup[nsub] = (uint32_t)i;  // uniform store
-                ++nsub;
-            }
+            const uint64_t m = ballot(ok);
+            if (ok) subgroup[nsub + lanes_below(m)] = (uint32_t)i;
+            nsub += (uint32_t)__popcll(m);
         }
+        group_sync();
         if (nsub == 0) return;
         for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt(v) = v;  // cnt is zero between steps
         H.sync();
@@ -957,8 +1068,10 @@ struct Worker {
         H.sync();
         for (uint32_t s = 0; s < nsub; ++s) {
             const ix_t* a = auts + (uint64_t)subgroup[s] * n;
+            for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.sb(v) = a[v];  // stage the images
+            H.sync();
             for (uint32_t v = (uint32_t)l; v < n; v += kWave) {
-                const uint32_t img = a[v];
+                const uint32_t img = H.sb(v);
                 if (img == v) continue;
                 // parents only point to smaller ids, so walks are shorter
                 // than n; the caps only bound a corrupted structure
@@ -1008,7 +1121,33 @@ struct Worker {
             }
             const uint32_t inc = wave_scan(d);
             const uint32_t o = run + inc - d;
-            if (i < n) {
+            if (i < n && d <= 8) {
+                row[i] = o;
+                label[i] = (ix_t)v;
+                const uint32_t a = H.row(v);
+                uint32_t kk[8];
+#pragma unroll
+                for (uint32_t t = 0; t < 8; ++t) kk[t] = t < d ? i * n + H.pos((uint32_t)col[a + t]) : 0xFFFFFFFFu;
+#pragma unroll
+                for (uint32_t k = 2; k <= 8; k <<= 1)
+#pragma unroll
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                        for (uint32_t q = 0; q < 8; ++q) {
+                            const uint32_t qj = q ^ j;
+                            if (qj > q) {
+                                const bool up = (q & k) == 0;
+                                const uint32_t x0 = kk[q], x1 = kk[qj];
+                                if ((x0 > x1) == up) {
+                                    kk[q] = x1;
+                                    kk[qj] = x0;
+                                }
+                            }
+                        }
+#pragma unroll
+                for (uint32_t t = 0; t < 8; ++t)
+                    if (t < d) keys[o + t] = kk[t];
+            } else if (i < n) {
                 row[i] = o;
                 label[i] = (ix_t)v;
                 const uint32_t a = H.row(v);
@@ -1044,6 +1183,23 @@ struct Worker {
                 const uint32_t r0 = row[i], r1 = row[i + 1];
                 if (r1 - r0 != d) {
                     differs = true;
+                } else if (d <= 8) {
+                    uint32_t ref[8], cur[8];
+#pragma unroll
+                    for (uint32_t t = 0; t < 8; ++t) {
+                        ref[t] = t < d ? keys[r0 + t] : 0xFFFFFFFFu;
+                        cur[t] = t < d ? (uint32_t)col[a + t] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t t = 0; t < 8; ++t) {
+                        if (t < d) {
+                            const uint32_t key = i * n + H.pos(cur[t]);
+                            bool found = false;
+#pragma unroll
+                            for (uint32_t z = 0; z < 8; ++z) found |= ref[z] == key;
+                            differs |= !found;
+                        }
+                    }
                 } else {
                     for (uint32_t t = 0; t < d && !differs; ++t) {
                         const uint32_t key = i * n + H.pos((uint32_t)col[a + t]);
@@ -1256,14 +1412,22 @@ struct Worker {
             cnt += arena_rep[r.arena_off + i] == arena_mem[r.arena_off + i] ? 1u : 0u;
         cnt = wave_sum(cnt);
         if (cnt < 2) return;
-        // claim a board slot
+        // take a free record
         int32_t slot = -1;
         if (lane_id() == 0) {
-            for (int b = 0; b < kBoard; ++b)
-                if (agent_cas(&P.board[b], 0u, 1u)) {
-                    slot = b;
-                    break;
+            const uint32_t nwr = (P.spec_records + 63) / 64;
+            for (uint32_t w = 0; w < nwr && slot < 0; ++w) {
+                uint64_t m = __hip_atomic_load(&P.slot_free[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (m) {
+                    const uint64_t bit = m & (~m + 1);
+                    const uint64_t old = atomicAnd(&P.slot_free[w], ~(unsigned long long)bit);
+                    if (old & bit) {
+                        slot = (int32_t)(w * 64u + (uint32_t)__builtin_ctzll(bit));
+                        break;
+                    }
+                    m = old & ~bit;
                 }
+            }
         }
         slot = __shfl(slot, 0, kWave);
         if (slot < 0) return;
@@ -1308,9 +1472,8 @@ struct Worker {
         group_sync();
         if (lane_id() == 0) {
             agent_store(&hd->epoch, epoch);
-            agent_store(&P.board[slot], 2u);
             __threadfence();
-            atomicOr(P.board_mask, (unsigned long long)(1ull << slot));
+            atomicOr(&P.board_mask[slot >> 6], (unsigned long long)(1ull << (slot & 63)));
         }
         group_sync();
     }
@@ -1365,16 +1528,16 @@ struct Worker {
         if (spec_slot < 0) return;
         uint32_t* state = spec_state();
         const uint32_t E = spec_epoch << 2;
-        if (lane_id() == 0) agent_store(&P.board[spec_slot], 1u);
-        group_sync();
         if (lane_id() == 0)
-            atomicAnd(P.board_mask, ~(unsigned long long)(1ull << spec_slot));
+            atomicAnd(&P.board_mask[spec_slot >> 6], ~(unsigned long long)(1ull << (spec_slot & 63)));
+        group_sync();
         for (uint32_t j = (uint32_t)lane_id(); j < spec_count; j += kWave) {
             agent_cas(&state[j], E | 0u, E | 3u);
             while (agent_load(&state[j]) == (E | 1u) && wall_clock64() <= deadline) __builtin_amdgcn_s_sleep(2);
         }
+        __threadfence();
         group_sync();
-        if (lane_id() == 0) agent_store(&P.board[spec_slot], 0u);
+        if (lane_id() == 0) atomicOr(&P.slot_free[spec_slot >> 6], (unsigned long long)(1ull << (spec_slot & 63)));
         group_sync();
         spec_slot = -1;
     }
@@ -1451,8 +1614,8 @@ struct Worker {
         budget = step_budget;
         spec_slot = -1;
         spec_maxn = P.maxn;
+        // counted in *pending by the host before the launch
         spec_pending = P.spec_min_n > 0 && n >= P.spec_min_n;
-        if (spec_pending && lane_id() == 0) atomicAdd(P.pending, 1u);
         init_partition(P.init_order, P.color, vbase);
         // search_tree_traversal (reference src/sequential.cpp:626-648)
         code = refine(-1);
@@ -1531,38 +1694,49 @@ struct Worker {
 template <class Hot>
 __device__ int help_once(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
-    uint64_t pub = 0;
-    uint32_t pend = 0;
-    if (l == 0) {
-        pub = __hip_atomic_load(P.board_mask, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (!pub) pend = agent_load(P.pending);
+    const uint32_t nwr = (P.spec_records + 63) / 64;
+    const uint64_t word = (uint32_t)l < nwr
+                              ? __hip_atomic_load(&P.board_mask[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                              : 0ull;
+    const uint64_t nz = ballot(word != 0ull);
+    if (!nz) {
+        uint32_t pend = 0;
+        if (l == 0) pend = agent_load(P.pending);
+        pend = __shfl(pend, 0, kWave);
+        return pend == 0 ? -1 : 0;
     }
-    pub = ((uint64_t)__shfl((uint32_t)(pub >> 32), 0, kWave) << 32) | __shfl((uint32_t)pub, 0, kWave);
-    pend = __shfl(pend, 0, kWave);
-    if (!pub) return pend == 0 ? -1 : 0;
     const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
-    // visit published slots starting at one that depends on the worker
-    const int rot = (int)(blockIdx.x % kBoard);
-    const uint64_t rotated = (pub >> rot) | (rot ? (pub << (kBoard - rot)) : 0ull);
+    // spread helpers: start at a word and bit that depend on the worker
+    const uint32_t rot = (blockIdx.x * 2654435761u) >> 16;
     uint32_t claim = 0xFFFFFFFFu, E = 0, slot_sel = 0;
-    if (l == 0) {
-        for (uint64_t m = rotated; m; m &= m - 1) {
-            const int slot = (__builtin_ctzll(m) + rot) % kBoard;
-            uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
-            SpecHeader* hd = (SpecHeader*)(rec + L.header);
-            const uint32_t e = agent_load(&hd->epoch) << 2;
-            const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
-            const uint32_t idx = atomicAdd(&hd->next, 1u);
-            if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
-                claim = idx;
-                E = e;
-                slot_sel = (uint32_t)slot;
-                break;
+    const uint32_t nnz_words = (uint32_t)__popcll(nz);
+    for (uint32_t t = 0; t < nnz_words && claim == 0xFFFFFFFFu; ++t) {
+        // t-th nonzero word after the rotation
+        uint64_t m = nz;
+        const uint32_t skip = (rot + t) % nnz_words;
+        for (uint32_t k = 0; k < skip; ++k) m &= m - 1;
+        const int wl = __builtin_ctzll(m);
+        const uint64_t bits = ((uint64_t)readlane((uint32_t)(word >> 32), wl) << 32) | readlane((uint32_t)word, wl);
+        if (l == 0) {
+            const uint32_t r2 = rot % 64u;
+            const uint64_t rb = (bits >> r2) | (r2 ? (bits << (64 - r2)) : 0ull);
+            for (uint64_t b = rb; b && claim == 0xFFFFFFFFu; b &= b - 1) {
+                const uint32_t slot = (uint32_t)wl * 64u + ((uint32_t)__builtin_ctzll(b) + r2) % 64u;
+                uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
+                SpecHeader* hd = (SpecHeader*)(rec + L.header);
+                const uint32_t e = agent_load(&hd->epoch) << 2;
+                const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
+                const uint32_t idx = atomicAdd(&hd->next, 1u);
+                if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
+                    claim = idx;
+                    E = e;
+                    slot_sel = slot;
+                }
             }
         }
+        claim = __shfl(claim, 0, kWave);
     }
-    claim = __shfl(claim, 0, kWave);
     if (claim == 0xFFFFFFFFu) return 0;
     E = __shfl(E, 0, kWave);
     slot_sel = __shfl(slot_sel, 0, kWave);
