@@ -38,8 +38,8 @@ struct DevBuf {
     }
 };
 
-constexpr int kClasses = 6;
-constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048};
+constexpr int kClasses = 7;
+constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048, 4096};
 
 template <int MAXN>
 hipError_t launch_lds(const KParams& P, int grid, hipStream_t s) {
@@ -55,6 +55,7 @@ hipError_t launch_class(int cls, const KParams& P, int grid, hipStream_t s) {
         case 3: return launch_lds<512>(P, grid, s);
         case 4: return launch_lds<1024>(P, grid, s);
         case 5: return launch_lds<2048>(P, grid, s);
+        case 6: return launch_lds<4096>(P, grid, s);
         default:
             hipLaunchKernelGGL(search_kernel_global, dim3(grid), dim3(kWave), 0, s, P);
             return hipGetLastError();
@@ -69,6 +70,7 @@ const void* class_kernel(int cls) {
         case 3: return (const void*)search_kernel_lds<512>;
         case 4: return (const void*)search_kernel_lds<1024>;
         case 5: return (const void*)search_kernel_lds<2048>;
+        case 6: return (const void*)search_kernel_lds<4096>;
         default: return (const void*)search_kernel_global;
     }
 }

@@ -201,21 +201,26 @@ __device__ __forceinline__ bool agent_cas(uint32_t* p, uint32_t expect, uint32_t
 }
 
 // ----------------------------------------------------------- hot storage
+// Row offsets are kept in LDS up to 2048 vertices; the 4096 class reads them
+// from global memory to fit the 160 KB of LDS.
 template <int MAXN>
 struct LdsHot {
     static constexpr int NW = MAXN / 64 + 2;
+    static constexpr int NROW = MAXN <= 2048 ? MAXN + 1 : 1;
     ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN], sa[MAXN], sb[MAXN], tv[MAXN], tcl[MAXN],
         sv[MAXN];
     uint32_t cnt[MAXN], hits[MAXN], c1[MAXN], c2[MAXN];
-    uint32_t rowl[MAXN + 1];
+    uint32_t rowl[NROW];
     uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
     uint32_t ctr[4];
+    const int64_t* rowp;
 };
 
 // Hot state in LDS: addresses are compile-time constants.
 template <int MAXN>
 struct HotL {
     static constexpr bool kLds = true;
+    static constexpr bool kRowLds = MAXN <= 2048;
     __device__ __forceinline__ static LdsHot<MAXN>& S() {
         __shared__ LdsHot<MAXN> s;
         return s;
@@ -239,10 +244,18 @@ struct HotL {
     __device__ __forceinline__ uint64_t* vmask() { return S().vmask; }
     __device__ __forceinline__ uint64_t* lasts() { return S().lasts; }
     __device__ __forceinline__ uint32_t* ctr() { return S().ctr; }
-    __device__ __forceinline__ uint32_t row(uint32_t v) { return S().rowl[v]; }
+    __device__ __forceinline__ uint32_t row(uint32_t v) {
+        if (kRowLds) return S().rowl[v];
+        const int64_t* rp = S().rowp;
+        return (uint32_t)(rp[v] - rp[0]);
+    }
     __device__ __forceinline__ void sync() { wave_sync(); }
     __device__ __forceinline__ void bind(const int64_t* rowp, uint32_t n, uint8_t*, uint32_t) {
-        for (uint32_t i = (uint32_t)lane_id(); i <= n; i += kWave) S().rowl[i] = (uint32_t)(rowp[i] - rowp[0]);
+        if (kRowLds) {
+            for (uint32_t i = (uint32_t)lane_id(); i <= n; i += kWave) S().rowl[i] = (uint32_t)(rowp[i] - rowp[0]);
+        } else {
+            S().rowp = rowp;
+        }
     }
 };
 
@@ -1778,6 +1791,9 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     bool graphs_left = true;
     uint32_t idle = 1;
     while (true) {
+        // help-first: root children of a published large graph are on the
+        // batch's critical path; a new graph is taken when there are none
+        if (P.spec_min_n && help_once(W, P) > 0) continue;
         if (graphs_left) {
             uint32_t item = 0;
             if (l == 0) item = atomicAdd(P.next_item, 1u);
