@@ -71,6 +71,8 @@ def build(with_reference=True):
     _run_all(cmds)
     lib = os.path.join(PKG, "libgcl.so")
     _run_all([[HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", lib] + objs])
+    probe = os.path.join(PKG, "hbm_probe")
+    _run_all([[HIPCC] + HIPFLAGS + [os.path.join(CSRC, "hbm_probe.hip"), "-o", probe]])
     cli = os.path.join(PKG, "gpu_canon_label")
     _run_all([[CXX] + CXXFLAGS + [os.path.join(CSRC, "gpu_canon_label.cpp"), "-o", cli,
                                   "-L" + PKG, "-lgcl", "-Wl,-rpath,$ORIGIN"]])
