@@ -201,15 +201,45 @@ __device__ __forceinline__ bool agent_cas(uint32_t* p, uint32_t expect, uint32_t
 }
 
 // ----------------------------------------------------------- hot storage
-// Row offsets are kept in LDS up to 2048 vertices; the 4096 class reads them
-// from global memory to fit the 160 KB of LDS.
+
+// 16-bit counter arrays stored as pairs in 32-bit words: loads and stores are
+// 16-bit, increments and compare-and-swap go through the containing word
+// (values stay below 2^16 since every count is at most n <= 65535).
+#define GCL_PACKED16(NAME, BASE)                                                                  \
+    __device__ __forceinline__ uint32_t NAME##_ld(uint32_t i) {                                   \
+        return ((volatile uint16_t*)(BASE))[i];                                                   \
+    }                                                                                             \
+    __device__ __forceinline__ void NAME##_st(uint32_t i, uint32_t v) {                           \
+        ((volatile uint16_t*)(BASE))[i] = (uint16_t)v;                                            \
+    }                                                                                             \
+    __device__ __forceinline__ uint32_t NAME##_add(uint32_t i) {                                  \
+        const uint32_t sh = (i & 1u) * 16u;                                                       \
+        return (atomicAdd(&(BASE)[i >> 1], 1u << sh) >> sh) & 0xFFFFu;                            \
+    }                                                                                             \
+    __device__ __forceinline__ bool NAME##_cas(uint32_t i, uint32_t expect, uint32_t desired) {   \
+        const uint32_t sh = (i & 1u) * 16u;                                                       \
+        uint32_t* w = &(BASE)[i >> 1];                                                            \
+        uint32_t old = *(volatile uint32_t*)w;                                                    \
+        for (int k = 0; k < 64; ++k) {                                                            \
+            if (((old >> sh) & 0xFFFFu) != expect) return false;                                  \
+            const uint32_t nw_ = (old & ~(0xFFFFu << sh)) | (desired << sh);                      \
+            const uint32_t got = atomicCAS(w, old, nw_);                                          \
+            if (got == old) return true;                                                          \
+            old = got;                                                                            \
+        }                                                                                         \
+        return false;                                                                             \
+    }
+
+// Row offsets are kept in LDS below 1024 vertices; the larger classes read
+// them from global memory, which leaves room for more workers per CU.
 template <int MAXN>
 struct LdsHot {
     static constexpr int NW = MAXN / 64 + 2;
-    static constexpr int NROW = MAXN <= 2048 ? MAXN + 1 : 1;
+    static constexpr int NROW = MAXN < 1024 ? MAXN + 1 : 1;
     ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN], sa[MAXN], sb[MAXN], tv[MAXN], tcl[MAXN],
         sv[MAXN];
-    uint32_t cnt[MAXN], hits[MAXN], c1[MAXN], c2[MAXN];
+    // 16-bit counters packed in pairs (atomics on the containing word)
+    uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1], c1[MAXN / 2 + 1], c2[MAXN / 2 + 1];
     uint32_t rowl[NROW];
     uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
     uint32_t ctr[4];
@@ -220,7 +250,7 @@ struct LdsHot {
 template <int MAXN>
 struct HotL {
     static constexpr bool kLds = true;
-    static constexpr bool kRowLds = MAXN <= 2048;
+    static constexpr bool kRowLds = MAXN < 1024;
     __device__ __forceinline__ static LdsHot<MAXN>& S() {
         __shared__ LdsHot<MAXN> s;
         return s;
@@ -235,10 +265,10 @@ struct HotL {
     __device__ __forceinline__ ix_t& tv(uint32_t i) { return S().tv[i]; }
     __device__ __forceinline__ ix_t& tcl(uint32_t i) { return S().tcl[i]; }
     __device__ __forceinline__ ix_t& sv(uint32_t i) { return S().sv[i]; }
-    __device__ __forceinline__ uint32_t& cnt(uint32_t i) { return S().cnt[i]; }
-    __device__ __forceinline__ uint32_t& hits(uint32_t i) { return S().hits[i]; }
-    __device__ __forceinline__ uint32_t& c1(uint32_t i) { return S().c1[i]; }
-    __device__ __forceinline__ uint32_t& c2(uint32_t i) { return S().c2[i]; }
+    GCL_PACKED16(cnt, S().cnt)
+    GCL_PACKED16(hits, S().hits)
+    GCL_PACKED16(c1, S().c1)
+    GCL_PACKED16(c2, S().c2)
     __device__ __forceinline__ uint64_t* active() { return S().active; }
     __device__ __forceinline__ uint64_t* nsmask() { return S().nsmask; }
     __device__ __forceinline__ uint64_t* vmask() { return S().vmask; }
@@ -286,10 +316,10 @@ struct HotG {
     __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[7 * n1 + i]; }
     __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[8 * n1 + i]; }
     __device__ __forceinline__ ix_t& sv(uint32_t i) { return h16[9 * n1 + i]; }
-    __device__ __forceinline__ uint32_t& cnt(uint32_t i) { return h32[i]; }
-    __device__ __forceinline__ uint32_t& hits(uint32_t i) { return h32[(n1 + 1) + i]; }
-    __device__ __forceinline__ uint32_t& c1(uint32_t i) { return h32[2 * (n1 + 1) + i]; }
-    __device__ __forceinline__ uint32_t& c2(uint32_t i) { return h32[3 * (n1 + 1) + i]; }
+    GCL_PACKED16(cnt, h32)
+    GCL_PACKED16(hits, h32 + (n1 + 2) / 2)
+    GCL_PACKED16(c1, h32 + 2 * ((n1 + 2) / 2))
+    GCL_PACKED16(c2, h32 + 3 * ((n1 + 2) / 2))
     __device__ __forceinline__ uint64_t* active() { return B().active; }
     __device__ __forceinline__ uint64_t* nsmask() { return B().nsmask; }
     __device__ __forceinline__ uint64_t* vmask() { return B().vmask; }
@@ -302,6 +332,14 @@ struct HotG {
         n1 = (uint64_t)maxn + 2;
         (void)base;
     }
+};
+
+// Register copies of the worker fields that every refinement step touches
+// (the worker object itself lives in LDS).
+struct Ctx {
+    uint32_t n, nw, ncells, nactive;
+    uint64_t steps, csr_bytes;
+    const int32_t* col;
 };
 
 // ------------------------------------------------------------------ worker
@@ -354,20 +392,20 @@ struct Worker {
             else m[p >> 6] &= ~(1ull << (p & 63));
         }
     }
-    __device__ __forceinline__ void add_active(uint32_t p) {
-        if (p < n && !test_bit(H.active(), p)) {
+    __device__ __forceinline__ void add_active(Ctx& X, uint32_t p) {
+        if (p < X.n && !test_bit(H.active(), p)) {
             set_bit_u(H.active(), p, true);
-            ++nactive;
+            ++X.nactive;
         }
     }
-    __device__ __forceinline__ void remove_active(uint32_t p) {
-        if (p < n && test_bit(H.active(), p)) {
+    __device__ __forceinline__ void remove_active(Ctx& X, uint32_t p) {
+        if (p < X.n && test_bit(H.active(), p)) {
             set_bit_u(H.active(), p, false);
-            --nactive;
+            --X.nactive;
         }
     }
     // first set bit at position >= start, or -1
-    __device__ __forceinline__ int32_t first_set_from(const uint64_t* m, uint32_t start) const {
+    __device__ __forceinline__ static int32_t first_set_from(const uint64_t* m, uint32_t start, uint32_t nw) {
         const int l = lane_id();
         for (uint32_t base = start >> 6; base < nw; base += kWave) {
             const uint32_t w = base + (uint32_t)l;
@@ -469,10 +507,10 @@ struct Worker {
                     const uint32_t v2 = init_order ? (uint32_t)init_order[vbase + p + 1] : p + 1;
                     is_last = color[vbase + v] != color[vbase + v2];
                 }
-                H.cnt(p) = 0;
-                H.hits(p) = 0;
-                H.c1(p) = 0;
-                H.c2(p) = 0;
+                H.cnt_st(p, 0);
+                H.hits_st(p, 0);
+                H.c1_st(p, 0);
+                H.c2_st(p, 0);
                 H.sa(p) = 0;
             }
             const uint64_t b = ballot(is_last);
@@ -491,28 +529,28 @@ struct Worker {
     // Refinement step, splitter {w} with deg(w) <= 64 (reference
     // src/partition_refinement.cpp:253-468, splitter.length == 1): cells
     // holding neighbours split into (neighbours, others).
-    __device__ void singleton_step(uint32_t w, uint32_t& longcode, int& hint) {
+    __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t w, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         const uint32_t e0 = H.row(w);
         const uint32_t deg = H.row(w + 1) - e0;
-        csr_bytes += 12 + 4ull * deg;
+        X.csr_bytes += 12 + 4ull * deg;
         uint32_t u = 0, c = 0, f = 0, r = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
-            u = (uint32_t)col[e0 + (uint32_t)l];
+            u = (uint32_t)X.col[e0 + (uint32_t)l];
             c = H.cend(H.pos(u));
             f = H.cfirst(c);
             ns = f != c;
         }
         if (ns) {
-            H.cnt(u) = 1;
-            r = atomicAdd(&H.hits(c), 1u);
+            H.cnt_st(u, 1);
+            r = H.hits_add(c);
         }
         H.sync();
         uint32_t k = 0, L = 0, q = 0;
         bool split = false, out = false;
         if (ns) {
-            k = H.hits(c);
+            k = H.hits_ld(c);
             L = c - f + 1;
             split = k < L;
             q = H.pos(u);
@@ -520,13 +558,13 @@ struct Worker {
         }
         if (split) {
             const uint32_t x = H.elem(f + r);
-            if (H.cnt(x) == 0) {
-                const uint32_t h = atomicAdd(&H.c1(c), 1u);
+            if (H.cnt_ld(x) == 0) {
+                const uint32_t h = H.c1_add(c);
                 H.tmp(f + h) = (ix_t)(f + r);
             }
         }
         uint32_t o = 0;
-        if (out) o = atomicAdd(&H.c2(c), 1u);
+        if (out) o = H.c2_add(c);
         H.sync();
         if (out) {  // move u into the front block over a non-neighbour
             const uint32_t hp = H.tmp(f + o);
@@ -542,9 +580,9 @@ struct Worker {
         if (split) H.cend(f + r) = (ix_t)(f + k - 1);
         uint32_t hkey = kNoKey;
         if (owner) {
-            H.hits(c) = 0;
-            H.c1(c) = 0;
-            H.c2(c) = 0;
+            H.hits_st(c, 0);
+            H.c1_st(c, 0);
+            H.c2_st(c, 0);
         }
         if (own) {
             H.cfirst(f + k - 1) = (ix_t)f;
@@ -563,15 +601,15 @@ struct Worker {
             if (L - k > 1) set_bit_atomic(H.nsmask(), f + k);
             set_bit_atomic(H.lasts(), f + k - 1);
         }
-        if (ns) H.cnt(u) = 0;
+        if (ns) H.cnt_st(u, 0);
         const uint64_t om = ballot(own);
         const uint32_t nsplit = (uint32_t)__popcll(om);
         if (nsplit == 0) {
             H.sync();
             return;
         }
-        ncells += nsplit;
-        nactive += nsplit;
+        X.ncells += nsplit;
+        X.nactive += nsplit;
         // boundary hash chain in position order (399-409): x = f+k-1 grows
         // with f; the hint is the candidate of the last split that has one
         const uint32_t x = f + k - 1;
@@ -600,7 +638,7 @@ struct Worker {
         for (uint32_t t = 0; t < kSmallCell; ++t) {
             if (t < L) {
                 const uint32_t v = H.elem(f + t);
-                const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
                 kk[t] = (kv << 16) | v;
             } else {
                 kk[t] = 0xFFFFFFFFu;
@@ -660,7 +698,7 @@ struct Worker {
     // |W| == 1), small cells one per lane, large ones by the whole wave; then
     // the hash chain and active-set rules over the pre-step non-singleton
     // cells in position order.
-    __device__ void general_step(uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
+    __device__ void general_step(Ctx& X, uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         const uint32_t s = e - p + 1;
         uint32_t* ctr = H.ctr();
@@ -677,15 +715,15 @@ struct Worker {
                 const uint32_t a = H.row(w), b = H.row(w + 1);
                 degsum += b - a;
                 for (uint32_t x = a; x < b; ++x) {
-                    const uint32_t u = (uint32_t)col[x];
-                    if (atomicAdd(&H.cnt(u), 1u) == 0u) {
+                    const uint32_t u = (uint32_t)X.col[x];
+                    if (H.cnt_add(u) == 0u) {
                         const uint32_t idx = atomicAdd(&ctr[0], 1u);
                         H.tv(idx) = (ix_t)u;
                     }
                 }
             }
         }
-        csr_bytes += 12ull * s + 4ull * wave_sum(degsum);
+        X.csr_bytes += 12ull * s + 4ull * wave_sum(degsum);
         H.sync();
         const uint32_t ntv = ctr[0];
         for (uint32_t base = 0; base < ntv; base += kWave) {
@@ -694,7 +732,7 @@ struct Worker {
                 const uint32_t u = H.tv(i);
                 const uint32_t c = H.cend(H.pos(u));
                 if (H.cfirst(c) != c) {
-                    if (atomicAdd(&H.hits(c), 1u) == 0u) {
+                    if (H.hits_add(c) == 0u) {
                         const uint32_t idx = atomicAdd(&ctr[1], 1u);
                         H.tcl(idx) = (ix_t)c;
                     }
@@ -712,7 +750,7 @@ struct Worker {
             if (i < ntc) {
                 c = H.tcl(i);
                 const uint32_t f = H.cfirst(c);
-                H.hits(c) = 0;
+                H.hits_st(c, 0);
                 if (c - f + 1 <= kSmallCell) {
                     nfrag_extra += regroup_small(f, c, s) - 1;
                     H.tcl(i) = (ix_t)f;  // now the old first
@@ -724,7 +762,7 @@ struct Worker {
             if (big) H.tcl(ntc + nbig + lanes_below(bm)) = (ix_t)c;
             nbig += (uint32_t)__popcll(bm);
         }
-        ncells += wave_sum(nfrag_extra);
+        X.ncells += wave_sum(nfrag_extra);
         H.sync();
         for (uint32_t j = 0; j < nbig; ++j) {
             const uint32_t c = H.tcl(ntc + j);
@@ -735,7 +773,7 @@ struct Worker {
                 const uint32_t i = base + (uint32_t)l;
                 if (i < L) {
                     const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
                     kmin = min(kmin, kv);
                     kmax = max(kmax, kv);
                 }
@@ -751,23 +789,23 @@ struct Worker {
                 continue;
             }
             const uint32_t R = kmax - kmin + 1;
-            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.c1(b) = 0;
+            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.c1_st(b, 0);
             H.sync();
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
                 if (i < L) {
                     const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
-                    H.sb(v) = (ix_t)atomicAdd(&H.c1(kv - kmin), 1u);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                    H.sb(v) = (ix_t)H.c1_add(kv - kmin);
                 }
             }
             H.sync();
             uint32_t run = 0, nfrag = 0;
             for (uint32_t base = 0; base < R; base += kWave) {
                 const uint32_t b = base + (uint32_t)l;
-                const uint32_t sz = b < R ? H.c1(b) : 0u;
+                const uint32_t sz = b < R ? H.c1_ld(b) : 0u;
                 const uint32_t inc = wave_scan(sz);
-                if (b < R) H.c2(b) = run + inc - sz;
+                if (b < R) H.c2_st(b, run + inc - sz);
                 run += readlane(inc, kWave - 1);
                 nfrag += (uint32_t)__popcll(ballot(sz != 0));
             }
@@ -776,8 +814,8 @@ struct Worker {
                 const uint32_t i = base + (uint32_t)l;
                 if (i < L) {
                     const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
-                    H.tmp(f + H.c2(kv - kmin) + H.sb(v)) = (ix_t)v;
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                    H.tmp(f + H.c2_ld(kv - kmin) + H.sb(v)) = (ix_t)v;
                 }
             }
             H.sync();
@@ -788,10 +826,10 @@ struct Worker {
                     const uint32_t v = H.tmp(P);
                     H.elem(P) = (ix_t)v;
                     H.pos(v) = (ix_t)P;
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt(v) : H.cnt(v);
+                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
                     const uint32_t b = kv - kmin;
-                    const uint32_t first = f + H.c2(b);
-                    const uint32_t last = first + H.c1(b) - 1;
+                    const uint32_t first = f + H.c2_ld(b);
+                    const uint32_t last = first + H.c1_ld(b) - 1;
                     H.cend(P) = (ix_t)last;
                     if (P == first) {
                         H.cfirst(last) = (ix_t)first;
@@ -799,11 +837,11 @@ struct Worker {
                     }
                 }
             }
-            ncells += nfrag - 1;
+            X.ncells += nfrag - 1;
             H.sync();
             for (uint32_t b = (uint32_t)l; b < R; b += kWave) {
-                H.c1(b) = 0;
-                H.c2(b) = 0;
+                H.c1_st(b, 0);
+                H.c2_st(b, 0);
             }
             if (l == 0) {
                 H.sa(f) = 2;
@@ -813,9 +851,9 @@ struct Worker {
         }
         // compact list of the pre-step non-singleton cells (position order)
         uint32_t nns = 0;
-        for (uint32_t base = 0; base < nw; base += kWave) {
+        for (uint32_t base = 0; base < X.nw; base += kWave) {
             const uint32_t w = base + (uint32_t)l;
-            uint64_t bits = w < nw ? H.nsmask()[w] : 0ull;
+            uint64_t bits = w < X.nw ? H.nsmask()[w] : 0ull;
             const uint32_t c0 = (uint32_t)__popcll(bits);
             const uint32_t inc = wave_scan(c0);
             uint32_t j = nns + inc - c0;
@@ -853,10 +891,10 @@ struct Worker {
                     const uint32_t LL = cold - fc + 1;
                     longcode = GCL_MASH(longcode, fc + K - 1);
                     if (test_bit(H.active(), fc) || K >= LL - K) {
-                        add_active(fc + K);
+                        add_active(X, fc + K);
                         if (LL - K == 1) hint = (int)(fc + K);
                     } else {
-                        add_active(fc);
+                        add_active(X, fc);
                         if (K == 1) hint = (int)fc;
                     }
                     continue;
@@ -865,7 +903,7 @@ struct Worker {
                 for (uint32_t q = fc; q <= cold;) {
                     const uint32_t last = H.cend(q);
                     const uint32_t size = last - q + 1;
-                    longcode = GCL_MASH(longcode, H.cnt(H.elem(q)) + q);
+                    longcode = GCL_MASH(longcode, H.cnt_ld(H.elem(q)) + q);
                     if (size > max_size) {
                         max_size = size;
                         max_first = q;
@@ -875,13 +913,13 @@ struct Worker {
                 const bool was_active = test_bit(H.active(), fc);
                 for (uint32_t q = (uint32_t)H.cend(fc) + 1; q <= cold;) {
                     const uint32_t last = H.cend(q);
-                    add_active(q);
+                    add_active(X, q);
                     if (last == q) hint = (int)q;
                     q = last + 1;
                 }
                 if (!was_active) {
-                    add_active(fc);
-                    remove_active(max_first);
+                    add_active(X, fc);
+                    remove_active(X, max_first);
                 }
             }
         }
@@ -908,7 +946,7 @@ struct Worker {
         H.sync();
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            if (i < ntv) H.cnt(H.tv(i)) = 0;
+            if (i < ntv) H.cnt_st(H.tv(i), 0);
         }
         H.sync();
     }
@@ -916,54 +954,66 @@ struct Worker {
     // Partition_s_refinement (reference src/partition_refinement.cpp:150-485)
     __device__ int refine(int32_t init_active) {
         const int l = lane_id();
+        Ctx X;
+        X.n = n;
+        X.nw = nw;
+        X.ncells = ncells;
+        X.nactive = nactive;
+        X.steps = steps;
+        X.csr_bytes = csr_bytes;
+        X.col = col;
         if (init_active < 0) {
             uint32_t total = 0;
-            for (uint32_t base = 0; base < n; base += kWave) {
+            for (uint32_t base = 0; base < X.n; base += kWave) {
                 const uint32_t p = base + (uint32_t)l;
-                const bool first = p < n && H.cfirst(H.cend(p)) == p;
+                const bool first = p < X.n && H.cfirst(H.cend(p)) == p;
                 const uint64_t b = ballot(first);
                 if (l == 0) H.active()[base >> 6] = b;
                 total += (uint32_t)__popcll(b);
             }
-            nactive = total;
+            X.nactive = total;
         } else {
-            add_active((uint32_t)init_active);
+            add_active(X, (uint32_t)init_active);
         }
         H.sync();
-        uint32_t longcode = ncells;
+        uint32_t longcode = X.ncells;
         int hint = 0;
         uint64_t bud = budget;
-        while (ncells < n && nactive > 0) {
+        while (X.ncells < X.n && X.nactive > 0) {
             if (bud == 0) {
                 status = GCL_E_INTERNAL;  // step budget exhausted
                 break;
             }
             --bud;
-            int32_t sp = first_set_from(H.active(), (uint32_t)hint);
-            if (sp < 0) sp = first_set_from(H.active(), 0);
+            int32_t sp = first_set_from(H.active(), (uint32_t)hint, X.nw);
+            if (sp < 0) sp = first_set_from(H.active(), 0, X.nw);
             const uint32_t p = (uint32_t)sp;
             set_bit_u(H.active(), p, false);
-            --nactive;
-            ++steps;
+            --X.nactive;
+            ++X.steps;
             const uint32_t e = H.cend(p);
             longcode = GCL_MASH(longcode, p + e);
             if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
             H.sync();
             if (e == p) {
                 const uint32_t w = H.elem(p);
-                if (H.row(w + 1) - H.row(w) <= (uint32_t)kWave) singleton_step(w, longcode, hint);
-                else general_step(p, e, longcode, hint);
+                if (H.row(w + 1) - H.row(w) <= (uint32_t)kWave) singleton_step(X, w, longcode, hint);
+                else general_step(X, p, e, longcode, hint);
             } else {
-                general_step(p, e, longcode, hint);
+                general_step(X, p, e, longcode, hint);
             }
         }
         budget = bud;
-        if (nactive) {
-            for (uint32_t w = (uint32_t)l; w < nw; w += kWave) H.active()[w] = 0;
-            nactive = 0;
+        if (X.nactive) {
+            for (uint32_t w = (uint32_t)l; w < X.nw; w += kWave) H.active()[w] = 0;
+            X.nactive = 0;
             H.sync();
         }
-        longcode = GCL_MASH(longcode, ncells);
+        ncells = X.ncells;
+        nactive = 0;
+        steps = X.steps;
+        csr_bytes = X.csr_bytes;
+        longcode = GCL_MASH(longcode, X.ncells);
         return GCL_CLEANUP(longcode);
     }
 
@@ -1004,7 +1054,7 @@ struct Worker {
     // src/partition_refinement.cpp:607-655): first non-singleton cell, members
     // in increasing vertex id.
     __device__ bool select_target() {
-        const int32_t f = first_set_from(H.nsmask(), 0);
+        const int32_t f = first_set_from(H.nsmask(), 0, nw);
         if (f < 0) return true;
         const int l = lane_id();
         const uint32_t c = H.cend((uint32_t)f);
@@ -1074,10 +1124,10 @@ struct Worker {
         }
         group_sync();
         if (nsub == 0) return;
-        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt(v) = v;  // cnt is zero between steps
+        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt_st(v, v);  // cnt is zero between steps
         H.sync();
         for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave)
-            H.cnt(arena_mem[r.arena_off + i]) = arena_rep[r.arena_off + i];
+            H.cnt_st(arena_mem[r.arena_off + i], arena_rep[r.arena_off + i]);
         H.sync();
         for (uint32_t s = 0; s < nsub; ++s) {
             const ix_t* a = auts + (uint64_t)subgroup[s] * n;
@@ -1091,18 +1141,18 @@ struct Worker {
                 uint32_t x = v, y = img;
                 for (uint32_t attempt = 0; attempt < n; ++attempt) {
                     for (uint32_t h = 0; h < n; ++h) {
-                        const uint32_t px = __hip_atomic_load(&H.cnt(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint32_t px = H.cnt_ld(x);
                         if (px == x) break;
                         x = px;
                     }
                     for (uint32_t h = 0; h < n; ++h) {
-                        const uint32_t py = __hip_atomic_load(&H.cnt(y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint32_t py = H.cnt_ld(y);
                         if (py == y) break;
                         y = py;
                     }
                     if (x == y) break;
                     const uint32_t lo = min(x, y), hi = max(x, y);
-                    if (atomicCAS(&H.cnt(hi), hi, lo) == hi) break;
+                    if (H.cnt_cas(hi, hi, lo)) break;
                     x = lo;
                     y = hi;
                 }
@@ -1111,11 +1161,11 @@ struct Worker {
         }
         for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave) {
             uint32_t x = arena_mem[r.arena_off + i];
-            for (uint32_t h = 0; h < n && H.cnt(x) != x; ++h) x = H.cnt(x);
+            for (uint32_t h = 0; h < n && H.cnt_ld(x) != x; ++h) x = H.cnt_ld(x);
             arena_rep[r.arena_off + i] = (ix_t)x;
         }
         H.sync();
-        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt(v) = 0;
+        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt_st(v, 0);
         group_sync();
     }
 
@@ -1665,10 +1715,10 @@ struct Worker {
         const uint64_t* rl = (const uint64_t*)(rec + L.root_lasts);
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) {
             H.elem(p) = re[p];
-            H.cnt(p) = 0;
-            H.hits(p) = 0;
-            H.c1(p) = 0;
-            H.c2(p) = 0;
+            H.cnt_st(p, 0);
+            H.hits_st(p, 0);
+            H.c1_st(p, 0);
+            H.c2_st(p, 0);
             H.sa(p) = 0;
         }
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) {
