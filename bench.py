@@ -5,7 +5,9 @@ n log-uniform in [16, 1024]; 50% G(n,p) with mean degree U[3,12], 30% random
 d-regular (d in {3,4,5}), 20% C_a x C_b (a, b in U[4,32]); each under a seeded
 random relabeling.  The batch is uploaded once (inputs resident in HBM); a
 step is one run of the persistent search launch (one kernel per size class,
-replayed from a captured hipGraph) and ends with a stream synchronize.
+each on its own stream, bracketed by HIP events on that stream; --hipgraph
+replays the same launches from a captured hipGraph instead) and ends with a
+stream synchronize.
 
 Printed as one JSON line with the throughput, the plain one-graph-per-launch
 rate on a sample of the same batch, the HBM roofline figure of the search
@@ -37,14 +39,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE), committed under profiles/."""
+def kernel_name(max_n):
+    return f"search_kernel_lds<{max_n}>" if max_n in (64, 128, 256, 512, 1024, 2048, 4096) else "search_kernel_global"
+
+
+def traffic_from_profile(kernel, graphs):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE, then WRITE_SIZE, each scaled by the copy-kernel calibration),
+    committed under profiles/ for the same workload."""
     path = os.path.join(REPO, "profiles", "search_batch_mix_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        t = json.load(f)
+    if t.get("graphs") != graphs:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 RE_TIME = re.compile(r"Algorithm ran in: (\d+)min, (\d+)s, (\d+)ms, (\d+)\S+s, (\d+)ns")
@@ -90,7 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--graphs", type=int, default=65536)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-graph", action="store_true", help="launch without hipGraph replay")
+    ap.add_argument("--hipgraph", action="store_true",
+                    help="replay the run as a captured hipGraph (no per-kernel events, so no roofline)")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--skip-baselines", action="store_true")
@@ -108,22 +120,36 @@ def main():
     log(f"generated {G} graphs, n total {int(batch.vtx_off[-1])}, nnz {int(batch.row_ptr[-1])} "
         f"in {time.time() - t0:.1f}s")
     db = gcl.DeviceBatch(batch, max_aut=10000)
-    use_graph = not args.no_graph
+    use_graph = args.hipgraph
     for w in range(args.warmup):
         k, t = db.run(use_graph=use_graph)
         log(f"warmup {w}: kernel {k:.1f} ms, wall {t:.1f} ms")
-    walls, kernels = [], []
+    walls, kernels, class_ms = [], [], []
     for s in range(args.steps):
         k, t = db.run(use_graph=use_graph)
         walls.append(t)
         kernels.append(k)
-        log(f"step {s}: kernel {k:.1f} ms, wall {t:.1f} ms")
+        cs = db.class_stats()  # after the run: events were recorded on each class stream
+        class_ms.append([c["kernel_ms"] for c in cs])
+        log(f"step {s}: kernel {k:.1f} ms, wall {t:.1f} ms; per class " +
+            ", ".join(f"n<={c['max_n']}: {c['kernel_ms']:.1f}" for c in cs))
     res = db.results()
     total_s = sum(walls) / 1e3
     value = args.steps * G / total_s
     ms_per_step = sum(walls) / args.steps
     kernel_ms = sum(kernels) / args.steps
-    achieved = res.csr_bytes / (kernel_ms / 1e3) / 1e9
+    # roofline of the dominant kernel: its algorithmic CSR bytes per launch
+    # over its average launch duration (HIP events on its own stream)
+    if use_graph:
+        # a replayed graph carries no events: whole launch instead
+        dom_name, dom_ms, dom_bytes = "all size classes (hipGraph replay)", kernel_ms, res.csr_bytes
+    else:
+        avg_class_ms = np.mean(np.array(class_ms), axis=0)
+        dom = int(np.argmax(avg_class_ms))
+        dom_name, dom_ms, dom_bytes = kernel_name(cs[dom]["max_n"]), float(avg_class_ms[dom]), cs[dom]["csr_bytes"]
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    _, _, _, dev_ns = db.graph_stats()
+    search_ms = dev_ns.astype(np.float64) / 1e6
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -141,8 +167,13 @@ def main():
                                "30% random d-regular d in {3,4,5}, 20% C_a x C_b a,b in U[4,32], seeded relabeling",
                    "graphs": G, "seed": args.seed, "hipgraph": use_graph, "workers": res.workers},
         "search_nodes_per_s": round(res.total_nodes / (kernel_ms / 1e3), 1),
+        "graph_search_ms": {"p50": round(float(np.percentile(search_ms, 50)), 3),
+                            "p99": round(float(np.percentile(search_ms, 99)), 3),
+                            "max": round(float(search_ms.max()), 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic_from_profile()},
+                     "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": traffic_from_profile(dom_name, G),
+                     "kernel": dom_name, "kernel_ms": round(dom_ms, 3), "algorithmic_bytes": int(dom_bytes)},
     }
     if not args.skip_baselines:
         # plain path: one launch per graph (allocation, copies, launch, sync)

@@ -313,6 +313,13 @@ gcl_status gcl_device_batch_graph_stats(gcl_device_batch* db, uint64_t* nodes, u
     return s;
 }
 
+gcl_status gcl_device_batch_class_stats(gcl_device_batch* db, gcl_class_stats* out, int32_t cap,
+                                        int32_t* count) {
+    if (!db) return fail(GCL_E_INVALID, "null handle");
+    if (cap > 0 && !out) return fail(GCL_E_INVALID, "null output array");
+    return device_batch_class_stats(reinterpret_cast<DeviceBatch*>(db), out, cap, count);
+}
+
 void gcl_device_batch_destroy(gcl_device_batch* db) { device_batch_destroy(reinterpret_cast<DeviceBatch*>(db)); }
 
 gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut, int32_t* out_label, uint64_t* out_nodes,

@@ -30,6 +30,7 @@ struct DeviceBatch;
 gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out);
 gcl_status device_batch_run(DeviceBatch* db, bool use_graph, float* ms);
 gcl_status device_batch_results(DeviceBatch* db, int32_t* out_label, DeviceGraphResult* res, gcl_batch_stats* st);
+gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32_t cap, int32_t* count);
 void device_batch_destroy(DeviceBatch* db);
 int32_t device_batch_size(const DeviceBatch* db);
 

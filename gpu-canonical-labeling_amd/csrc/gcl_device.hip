@@ -132,6 +132,8 @@ struct DeviceBatch {
         uint32_t spec_big = 0;   // graphs of the class that may publish
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;  // timing of the class kernel on its stream
+        std::vector<int32_t> graphs;            // host copy of the class's items
     };
     int32_t G = 0;
     int64_t total_n = 0, total_nnz = 0;
@@ -143,12 +145,15 @@ struct DeviceBatch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t workers = 0;
+    bool timed_run = false;  // last run carried per-class events
 
     ~DeviceBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto& c : classes) {
             if (c.done) (void)hipEventDestroy(c.done);
+            if (c.t0) (void)hipEventDestroy(c.t0);
+            if (c.t1) (void)hipEventDestroy(c.t1);
             if (c.stream) (void)hipStreamDestroy(c.stream);
         }
         if (ev0) (void)hipEventDestroy(ev0);
@@ -291,6 +296,9 @@ struct DeviceBatch {
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
             GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
             GCL_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
+            GCL_HIP(hipEventCreate(&C.t0));
+            GCL_HIP(hipEventCreate(&C.t1));
+            C.graphs = items_h[cls];
             KParams& P = C.P;
             P.num_items = (int32_t)cnt;
             P.items = (const int32_t*)items.p + item_off;
@@ -334,19 +342,29 @@ struct DeviceBatch {
         return GCL_OK;
     }
 
-    // reset counters, fork one stream per size class, join
-    hipError_t enqueue() {
+    // reset counters, fork one stream per size class, join; `timed` brackets
+    // each class kernel with events on its stream (HIP cannot time events
+    // recorded inside a captured graph, so the replayed graph has none)
+    hipError_t enqueue(bool timed) {
         hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p,
                            (const unsigned int*)ctr_init.p, 64);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if ((e = hipEventRecord(start, main)) != hipSuccess) return e;
-        for (auto& c : classes) {
+        // largest class first, so that its workers take the CUs first; the
+        // joins go to the main stream only after every launch, since streams
+        // share hardware queues and a wait queued ahead of a launch on the
+        // same queue would hold that launch back
+        for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
+            Cls& c = *it;
             if ((e = hipStreamWaitEvent(c.stream, start, 0)) != hipSuccess) return e;
+            if (timed && (e = hipEventRecord(c.t0, c.stream)) != hipSuccess) return e;
             if ((e = launch_class(c.cls, c.P, (int)c.grid, c.stream)) != hipSuccess) return e;
+            if (timed && (e = hipEventRecord(c.t1, c.stream)) != hipSuccess) return e;
             if ((e = hipEventRecord(c.done, c.stream)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(main, c.done, 0)) != hipSuccess) return e;
         }
+        for (auto& c : classes)
+            if ((e = hipStreamWaitEvent(main, c.done, 0)) != hipSuccess) return e;
         return hipSuccess;
     }
 
@@ -354,7 +372,7 @@ struct DeviceBatch {
         GCL_HIP(hipSetDevice(0));
         if (use_graph && !exec) {
             GCL_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeGlobal));
-            hipError_t e = enqueue();
+            hipError_t e = enqueue(false);
             hipGraph_t gr = nullptr;
             hipError_t e2 = hipStreamEndCapture(main, &gr);
             if (e != hipSuccess) return fail(GCL_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
@@ -366,15 +384,45 @@ struct DeviceBatch {
         if (use_graph) {
             GCL_HIP(hipGraphLaunch(exec, main));
         } else {
-            hipError_t e = enqueue();
+            hipError_t e = enqueue(true);
             if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
         }
         GCL_HIP(hipEventRecord(ev1, main));
+        timed_run = !use_graph;
         hipError_t se = hipStreamSynchronize(main);
         if (se != hipSuccess) return fail(GCL_E_DEVICE, "search kernel failed: %s", hipGetErrorString(se));
         float t = 0.f;
         GCL_HIP(hipEventElapsedTime(&t, ev0, ev1));
         if (ms) *ms = t;
+        return GCL_OK;
+    }
+
+    // Per size class: bound, graphs, workers, last run's kernel time (events
+    // on the class's stream) and the class's totals from the result records.
+    gcl_status class_stats(gcl_class_stats* out, int32_t cap, int32_t* count) {
+        GCL_HIP(hipSetDevice(0));
+        std::vector<DeviceGraphResult> res((size_t)std::max(G, 1));
+        if (G > 0)
+            GCL_HIP(hipMemcpy(res.data(), this->out.p, sizeof(DeviceGraphResult) * (size_t)G,
+                              hipMemcpyDeviceToHost));
+        const int32_t k = (int32_t)classes.size();
+        if (count) *count = k;
+        for (int32_t i = 0; i < std::min(k, cap); ++i) {
+            const Cls& c = classes[(size_t)i];
+            gcl_class_stats& o = out[i];
+            memset(&o, 0, sizeof(o));
+            o.max_n = (int32_t)c.P.maxn;
+            o.graphs = c.count;
+            o.workers = (int32_t)c.grid;
+            float t = -1.f;
+            if (timed_run) GCL_HIP(hipEventElapsedTime(&t, c.t0, c.t1));
+            o.kernel_ms = t;
+            for (int32_t g : c.graphs) {
+                o.nodes += res[(size_t)g].nodes;
+                o.splitter_steps += res[(size_t)g].splitter_steps;
+                o.csr_bytes += res[(size_t)g].csr_bytes;
+            }
+        }
         return GCL_OK;
     }
 
@@ -429,6 +477,10 @@ gcl_status device_batch_run(DeviceBatch* db, bool use_graph, float* ms) { return
 
 gcl_status device_batch_results(DeviceBatch* db, int32_t* out_label, DeviceGraphResult* res, gcl_batch_stats* st) {
     return db->results(out_label, res, st);
+}
+
+gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32_t cap, int32_t* count) {
+    return db->class_stats(out, cap, count);
 }
 
 void device_batch_destroy(DeviceBatch* db) { delete db; }

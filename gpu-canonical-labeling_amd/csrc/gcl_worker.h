@@ -115,7 +115,7 @@ __host__ __device__ inline SpecLayout make_spec_layout(uint32_t maxn, uint32_t s
     L.state = o; o = align256(o + 4 * (uint64_t)slots);
     L.code = o; o = align256(o + 4 * (uint64_t)slots);
     L.ncells = o; o = align256(o + 4 * (uint64_t)slots);
-    L.steps = o; o = align256(o + 8 * (uint64_t)slots);
+    L.steps = o; o = align256(o + 16 * (uint64_t)slots);  // (splitter steps, CSR bytes) per item
     L.elem = o; o = align256(o + 2 * n1 * slots);
     L.lasts = o; o = align256(o + 8 * nw1 * slots);
     L.total = o;
@@ -189,9 +189,12 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x) {  // inclusive
     return x;
 }
 __device__ __forceinline__ uint32_t readlane(uint32_t v, int i) { return __builtin_amdgcn_readlane(v, i); }
-__device__ __forceinline__ uint32_t agent_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// polling load: no cache invalidation per poll; pair with agent_acquire()
+// once the awaited value has been seen
+__device__ __forceinline__ uint32_t agent_peek(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void agent_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void agent_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1560,19 +1563,21 @@ struct Worker {
         won = __shfl(won, 0, kWave);
         if (won) return false;  // withdrawn: refine it here
         // a helper has it: wait until done, then copy its partition
-        while (agent_load(&state[j]) != (E | 2u)) {
+        while (agent_peek(&state[j]) != (E | 2u)) {
             if (wall_clock64() > deadline) {
                 status = GCL_E_INTERNAL;  // a helper did not finish in time
                 return false;
             }
             __builtin_amdgcn_s_sleep(2);
         }
+        agent_acquire();
         const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
         const uint64_t* il = (const uint64_t*)(spec_rec + SL.lasts) + (uint64_t)j * (P_nw1());
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = il[w];
         code = (int)((const uint32_t*)(spec_rec + SL.code))[j];
-        steps += ((const uint64_t*)(spec_rec + SL.steps))[j];
+        steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
+        csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
         H.sync();
         positions_from_elem();
         rebuild_from_lasts();
@@ -1596,7 +1601,7 @@ struct Worker {
         group_sync();
         for (uint32_t j = (uint32_t)lane_id(); j < spec_count; j += kWave) {
             agent_cas(&state[j], E | 0u, E | 3u);
-            while (agent_load(&state[j]) == (E | 1u) && wall_clock64() <= deadline) __builtin_amdgcn_s_sleep(2);
+            while (agent_peek(&state[j]) == (E | 1u) && wall_clock64() <= deadline) __builtin_amdgcn_s_sleep(2);
         }
         __threadfence();
         group_sync();
@@ -1731,7 +1736,7 @@ struct Worker {
         positions_from_elem();
         rebuild_from_lasts();
         const uint32_t v = ((const ix_t*)(rec + L.vert))[j];
-        const uint64_t s0 = steps;
+        const uint64_t s0 = steps, b0 = csr_bytes;
         budget = step_budget;
         status = 0;
         individualize_and_refine(v, tcf);
@@ -1743,7 +1748,8 @@ struct Worker {
         if (lane_id() == 0) {
             ((uint32_t*)(rec + L.code))[j] = (uint32_t)code;
             ((uint32_t*)(rec + L.ncells))[j] = ncells;
-            ((uint64_t*)(rec + L.steps))[j] = steps - s0;
+            ((uint64_t*)(rec + L.steps))[2 * j] = steps - s0;
+            ((uint64_t*)(rec + L.steps))[2 * j + 1] = csr_bytes - b0;
         }
         __threadfence();
         group_sync();
@@ -1759,12 +1765,12 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
     const uint32_t nwr = (P.spec_records + 63) / 64;
     const uint64_t word = (uint32_t)l < nwr
-                              ? __hip_atomic_load(&P.board_mask[l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                              ? __hip_atomic_load(&P.board_mask[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : 0ull;
     const uint64_t nz = ballot(word != 0ull);
     if (!nz) {
         uint32_t pend = 0;
-        if (l == 0) pend = agent_load(P.pending);
+        if (l == 0) pend = agent_peek(P.pending);
         pend = __shfl(pend, 0, kWave);
         return pend == 0 ? -1 : 0;
     }
@@ -1787,7 +1793,9 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
                 const uint32_t slot = (uint32_t)wl * 64u + ((uint32_t)__builtin_ctzll(b) + r2) % 64u;
                 uint8_t* rec = P.spec + (uint64_t)slot * P.spec_stride;
                 SpecHeader* hd = (SpecHeader*)(rec + L.header);
-                const uint32_t e = agent_load(&hd->epoch) << 2;
+                // the state word carries the epoch, so the claim below
+                // (acquire) validates this relaxed read
+                const uint32_t e = agent_peek(&hd->epoch) << 2;
                 const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
                 const uint32_t idx = atomicAdd(&hd->next, 1u);

@@ -59,6 +59,12 @@ class _BatchStats(C.Structure):
                 ("workers", C.c_int32)]
 
 
+class _ClassStats(C.Structure):
+    _fields_ = [("max_n", C.c_int32), ("graphs", C.c_int32), ("workers", C.c_int32), ("pad", C.c_int32),
+                ("kernel_ms", C.c_double), ("nodes", C.c_uint64), ("splitter_steps", C.c_uint64),
+                ("csr_bytes", C.c_uint64)]
+
+
 _lib = None
 
 # exported symbols and their signatures (restype, argtypes)
@@ -90,6 +96,7 @@ _SIGS = {
     "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_device_batch_destroy": (None, [C.c_void_p]),
+    "gcl_device_batch_class_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_graph_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_gen_mix_batch": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "gcl_owned_batch_free": (None, [C.c_void_p]),
@@ -421,6 +428,16 @@ class DeviceBatch:
         a = [np.zeros(G, np.uint64) for _ in range(4)]
         _check(lib().gcl_device_batch_graph_stats(self.h, *[_ptr(x) for x in a]))
         return tuple(x[:self.batch.num_graphs] for x in a)
+
+    def class_stats(self):
+        """Per size class of the last run: dicts with max_n, graphs, workers,
+        kernel_ms (HIP events on the class's stream), nodes, splitter_steps
+        and csr_bytes."""
+        cnt = C.c_int32(0)
+        _check(lib().gcl_device_batch_class_stats(self.h, None, 0, C.byref(cnt)))
+        arr = (_ClassStats * max(cnt.value, 1))()
+        _check(lib().gcl_device_batch_class_stats(self.h, arr, cnt.value, None))
+        return [{f: getattr(arr[i], f) for f, _ in _ClassStats._fields_ if f != "pad"} for i in range(cnt.value)]
 
     def close(self):
         if self.h:

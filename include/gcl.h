@@ -174,6 +174,24 @@ gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, ui
  * time of the graph's search in nanoseconds. */
 gcl_status gcl_device_batch_graph_stats(gcl_device_batch* db, uint64_t* nodes, uint64_t* automorphisms,
                                         uint64_t* splitter_steps, uint64_t* device_ns);
+/* Per size class of the batch (one persistent kernel each), for the last
+ * run: the class bound on n, graphs, workers, the kernel's time from HIP
+ * events recorded on the class's own stream (-1 after a hipGraph replay,
+ * which carries no events), and the class's search nodes,
+ * splitter steps and algorithmic CSR bytes.  Fills min(cap, classes)
+ * entries; *count (optional) receives the number of classes. */
+typedef struct gcl_class_stats {
+    int32_t max_n;
+    int32_t graphs;
+    int32_t workers;
+    int32_t pad;
+    double kernel_ms;
+    uint64_t nodes;
+    uint64_t splitter_steps;
+    uint64_t csr_bytes;
+} gcl_class_stats;
+gcl_status gcl_device_batch_class_stats(gcl_device_batch* db, gcl_class_stats* out, int32_t cap,
+                                        int32_t* count);
 void gcl_device_batch_destroy(gcl_device_batch* db);
 
 /* Canonical keys of one graph from a labeling: out_keys[nnz] = sorted

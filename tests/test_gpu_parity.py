@@ -145,3 +145,25 @@ def test_gpu_speculation_keeps_traversal_exact(monkeypatch):
                 r = gcl.canon_label(g, max_aut=max_aut, engine="gpu")
                 assert np.array_equal(r.keys, o["keys"]), g.n
                 assert (r.nodes, r.automorphisms) == (o["nodes"], o["automorphisms"]), (g.n, spec, max_aut)
+
+
+def test_gpu_device_batch_counters_are_deterministic():
+    # work done by idle workers on a graph's root children is credited to
+    # that graph, so per-graph steps and nodes do not depend on who did it
+    graphs = _mixed_graphs(200, 5) + [gcl.random_regular(600, 3, 8), gcl.torus(24, 30)]
+    db = gcl.DeviceBatch(gcl.Batch(graphs), max_aut=64)
+    runs = []
+    # eager runs, then several replays of the captured hipGraph: each replay
+    # must agree with the eager result
+    for rep, use_graph in enumerate([False, False, True, True, True]):
+        _log(f"device batch run {rep} hipgraph={use_graph}")
+        db.run(use_graph=use_graph)
+        nodes, auts, steps, _ = db.graph_stats()
+        runs.append((nodes.copy(), auts.copy(), steps.copy(), db.results().labels.copy()))
+    cs = db.class_stats()
+    assert sum(c["graphs"] for c in cs) == len(graphs)
+    assert sum(c["splitter_steps"] for c in cs) == int(runs[-1][2].sum())
+    db.close()
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            assert np.array_equal(a, b)
