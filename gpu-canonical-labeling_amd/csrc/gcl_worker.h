@@ -49,12 +49,16 @@ __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~u
 // Per-worker global scratch.
 struct Layout {
     uint64_t snap, seq, first_seq, best_seq, lv, arena_mem, arena_rep, firstcode, canon_code, first_label,
-        best_label, first_keys, best_keys, first_row, best_row, subgroup, auts, hot16, hot32, total;
+        best_label, first_keys, best_keys, first_row, best_row, subgroup, auts, hot16, hot32, hot64, total;
     uint32_t levels;  // snapshot levels available
 };
 
-constexpr int kHot16 = 10;  // elem pos cend cfirst tmp sa sb tv tcl sv
-constexpr int kHot32 = 4;   // cnt hits c1 c2
+constexpr int kHot16 = 7;  // elem pos cend cfirst tmp(=sb) tv(=c1) tcl(=c2)
+constexpr int kHot32 = 2;  // cnt hits
+constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
+// element stride of the global hot arrays: even, so that the 16-bit arrays
+// that double as packed counters start on a 32-bit word
+__host__ __device__ inline uint64_t hot_stride(uint32_t maxn) { return ((uint64_t)maxn + 2 + 63) & ~uint64_t(63); }
 
 __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, uint32_t ring, uint64_t arena,
                                               uint32_t levels, bool hot_global) {
@@ -80,8 +84,9 @@ __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, u
     L.best_row = o; o = align256(o + 4 * n1);
     L.subgroup = o; o = align256(o + 4 * (uint64_t)ring);
     L.auts = o; o = align256(o + 2 * (uint64_t)ring * maxn);
-    L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * n1);
-    L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (n1 + 1));
+    L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * hot_stride(maxn));
+    L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (hot_stride(maxn) / 2 + 1));
+    L.hot64 = o; if (hot_global) o = align256(o + 8ull * kHot64 * nw1);
     L.total = o;
     return L;
 }
@@ -239,12 +244,18 @@ template <int MAXN>
 struct LdsHot {
     static constexpr int NW = MAXN / 64 + 2;
     static constexpr int NROW = MAXN < 1024 ? MAXN + 1 : 1;
-    ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN], sa[MAXN], sb[MAXN], tv[MAXN], tcl[MAXN],
-        sv[MAXN];
+    // tmp is also sb: the staging area of a large-cell regroup and the list
+    // of non-singleton cells are used one after the other
+    ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN];
     // 16-bit counters packed in pairs (atomics on the containing word)
-    uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1], c1[MAXN / 2 + 1], c2[MAXN / 2 + 1];
+    uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1];
+    // touched-vertex / touched-cell lists of a general step; the same words
+    // are the zero-kept counters c1 / c2 of a singleton step (a general step
+    // clears the entries it wrote)
+    uint32_t tvw[MAXN / 2 + 1], tclw[MAXN / 2 + 1];
     uint32_t rowl[NROW];
     uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
+    uint64_t split[NW], oldend[NW];  // per general step: split cells (by first), their old lasts
     uint32_t ctr[4];
     const int64_t* rowp;
 };
@@ -263,15 +274,15 @@ struct HotL {
     __device__ __forceinline__ ix_t& cend(uint32_t i) { return S().cend[i]; }
     __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return S().cfirst[i]; }
     __device__ __forceinline__ ix_t& tmp(uint32_t i) { return S().tmp[i]; }
-    __device__ __forceinline__ ix_t& sa(uint32_t i) { return S().sa[i]; }
-    __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().sb[i]; }
-    __device__ __forceinline__ ix_t& tv(uint32_t i) { return S().tv[i]; }
-    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return S().tcl[i]; }
-    __device__ __forceinline__ ix_t& sv(uint32_t i) { return S().sv[i]; }
+    __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().tmp[i]; }
+    __device__ __forceinline__ ix_t& tv(uint32_t i) { return ((ix_t*)S().tvw)[i]; }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return ((ix_t*)S().tclw)[i]; }
     GCL_PACKED16(cnt, S().cnt)
     GCL_PACKED16(hits, S().hits)
-    GCL_PACKED16(c1, S().c1)
-    GCL_PACKED16(c2, S().c2)
+    GCL_PACKED16(c1, S().tvw)
+    GCL_PACKED16(c2, S().tclw)
+    __device__ __forceinline__ uint64_t* split() { return S().split; }
+    __device__ __forceinline__ uint64_t* oldend() { return S().oldend; }
     __device__ __forceinline__ uint64_t* active() { return S().active; }
     __device__ __forceinline__ uint64_t* nsmask() { return S().nsmask; }
     __device__ __forceinline__ uint64_t* vmask() { return S().vmask; }
@@ -303,7 +314,8 @@ struct HotG {
     static constexpr bool kLds = false;
     ix_t* h16;
     uint32_t* h32;
-    uint64_t n1;
+    uint64_t* h64;
+    uint64_t n1, nw1;  // array stride (hot_stride), bitmap words
     const int64_t* rowp;
     __device__ __forceinline__ static LdsBits& B() {
         __shared__ LdsBits b;
@@ -314,15 +326,15 @@ struct HotG {
     __device__ __forceinline__ ix_t& cend(uint32_t i) { return h16[2 * n1 + i]; }
     __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return h16[3 * n1 + i]; }
     __device__ __forceinline__ ix_t& tmp(uint32_t i) { return h16[4 * n1 + i]; }
-    __device__ __forceinline__ ix_t& sa(uint32_t i) { return h16[5 * n1 + i]; }
-    __device__ __forceinline__ ix_t& sb(uint32_t i) { return h16[6 * n1 + i]; }
-    __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[7 * n1 + i]; }
-    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[8 * n1 + i]; }
-    __device__ __forceinline__ ix_t& sv(uint32_t i) { return h16[9 * n1 + i]; }
+    __device__ __forceinline__ ix_t& sb(uint32_t i) { return h16[4 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[5 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[6 * n1 + i]; }
     GCL_PACKED16(cnt, h32)
-    GCL_PACKED16(hits, h32 + (n1 + 2) / 2)
-    GCL_PACKED16(c1, h32 + 2 * ((n1 + 2) / 2))
-    GCL_PACKED16(c2, h32 + 3 * ((n1 + 2) / 2))
+    GCL_PACKED16(hits, h32 + (n1 / 2 + 1))
+    GCL_PACKED16(c1, (uint32_t*)(h16 + 5 * n1))
+    GCL_PACKED16(c2, (uint32_t*)(h16 + 6 * n1))
+    __device__ __forceinline__ uint64_t* split() { return h64; }
+    __device__ __forceinline__ uint64_t* oldend() { return h64 + nw1; }
     __device__ __forceinline__ uint64_t* active() { return B().active; }
     __device__ __forceinline__ uint64_t* nsmask() { return B().nsmask; }
     __device__ __forceinline__ uint64_t* vmask() { return B().vmask; }
@@ -332,7 +344,8 @@ struct HotG {
     __device__ __forceinline__ void sync() { group_sync(); }
     __device__ __forceinline__ void bind(const int64_t* rp, uint32_t, uint8_t* base, uint32_t maxn) {
         rowp = rp;
-        n1 = (uint64_t)maxn + 2;
+        n1 = hot_stride(maxn);
+        nw1 = (uint64_t)maxn / 64 + 2;
         (void)base;
     }
 };
@@ -514,7 +527,6 @@ struct Worker {
                 H.hits_st(p, 0);
                 H.c1_st(p, 0);
                 H.c2_st(p, 0);
-                H.sa(p) = 0;
             }
             const uint64_t b = ballot(is_last);
             if (l == 0) H.lasts()[base >> 6] = b;
@@ -522,6 +534,8 @@ struct Worker {
         for (uint32_t w = (uint32_t)l; w < nw; w += kWave) {
             H.active()[w] = 0;
             H.vmask()[w] = 0;
+            H.split()[w] = 0;
+            H.oldend()[w] = 0;
         }
         nactive = 0;
         H.sync();
@@ -632,7 +646,8 @@ struct Worker {
 
     // Regroups one touched cell [f, c] of at most kSmallCell members by its
     // key, entirely in this lane (sorting network in registers).  Returns the
-    // number of fragments; records the cell's state for the hash pass.
+    // number of fragments; a cell that splits is marked for the hash pass
+    // (bit f of split, bit c of oldend).
     static constexpr uint32_t kSmallCell = 8;
     __device__ __forceinline__ uint32_t regroup_small(uint32_t f, uint32_t c, uint32_t s) {
         const uint32_t L = c - f + 1;
@@ -669,11 +684,7 @@ struct Worker {
 #pragma unroll
         for (uint32_t t = 0; t < kSmallCell; ++t)
             if (t + 1 == L) kmax = kk[t] >> 16;
-        if (kmin == kmax) {
-            H.sa(f) = 1;
-            H.sv(f) = (ix_t)((s == 1) ? 1u - kmin : kmin);
-            return 1;
-        }
+        if (kmin == kmax) return 1;
         uint32_t nfrag = 0, end = 0;
 #pragma unroll
         for (int t = (int)kSmallCell - 1; t >= 0; --t) {
@@ -690,8 +701,8 @@ struct Worker {
                 }
             }
         }
-        H.sa(f) = 2;
-        H.sv(f) = (ix_t)c;
+        set_bit_atomic(H.split(), f);
+        set_bit_atomic(H.oldend(), c);
         return nfrag;
     }
 
@@ -744,7 +755,8 @@ struct Worker {
         }
         H.sync();
         const uint32_t ntc = ctr[1];
-        // small cells: one lane each; large ones listed after the touched list
+        // small cells: one lane each; large ones are compacted to the front
+        // of the touched list (entry nbig + rank <= i was already read)
         uint32_t nbig = 0, nfrag_extra = 0;
         for (uint32_t base = 0; base < ntc; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
@@ -756,19 +768,20 @@ struct Worker {
                 H.hits_st(c, 0);
                 if (c - f + 1 <= kSmallCell) {
                     nfrag_extra += regroup_small(f, c, s) - 1;
-                    H.tcl(i) = (ix_t)f;  // now the old first
                 } else {
                     big = true;
                 }
             }
             const uint64_t bm = ballot(big);
-            if (big) H.tcl(ntc + nbig + lanes_below(bm)) = (ix_t)c;
+            if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
             nbig += (uint32_t)__popcll(bm);
         }
         X.ncells += wave_sum(nfrag_extra);
         H.sync();
+        // hits is all zero from here on: it holds the buckets of the
+        // counting sort below (indices 0..R-1, cleared after each cell)
         for (uint32_t j = 0; j < nbig; ++j) {
-            const uint32_t c = H.tcl(ntc + j);
+            const uint32_t c = H.tcl(j);
             const uint32_t f = H.cfirst(c);
             const uint32_t L = c - f + 1;
             uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
@@ -783,42 +796,36 @@ struct Worker {
             }
             kmin = wave_min(kmin);
             kmax = wave_max(kmax);
-            if (kmin == kmax) {
-                if (l == 0) {
-                    H.sa(f) = 1;
-                    H.sv(f) = (ix_t)((s == 1) ? 1u - kmin : kmin);
-                }
-                H.sync();
-                continue;
-            }
+            if (kmin == kmax) continue;  // uniform: nothing moves
             const uint32_t R = kmax - kmin + 1;
-            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.c1_st(b, 0);
-            H.sync();
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
                 if (i < L) {
                     const uint32_t v = H.elem(f + i);
                     const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    H.sb(v) = (ix_t)H.c1_add(kv - kmin);
+                    H.hits_add(kv - kmin);
                 }
             }
             H.sync();
+            // bucket sizes -> exclusive offsets, in place
             uint32_t run = 0, nfrag = 0;
             for (uint32_t base = 0; base < R; base += kWave) {
                 const uint32_t b = base + (uint32_t)l;
-                const uint32_t sz = b < R ? H.c1_ld(b) : 0u;
+                const uint32_t sz = b < R ? H.hits_ld(b) : 0u;
                 const uint32_t inc = wave_scan(sz);
-                if (b < R) H.c2_st(b, run + inc - sz);
+                if (b < R) H.hits_st(b, run + inc - sz);
                 run += readlane(inc, kWave - 1);
                 nfrag += (uint32_t)__popcll(ballot(sz != 0));
             }
             H.sync();
+            // each member takes the next slot of its bucket (the order inside
+            // a fragment is unobservable); offsets end as bucket ends
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
                 if (i < L) {
                     const uint32_t v = H.elem(f + i);
                     const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    H.tmp(f + H.c2_ld(kv - kmin) + H.sb(v)) = (ix_t)v;
+                    H.tmp(f + H.hits_add(kv - kmin)) = (ix_t)v;
                 }
             }
             H.sync();
@@ -831,8 +838,8 @@ struct Worker {
                     H.pos(v) = (ix_t)P;
                     const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
                     const uint32_t b = kv - kmin;
-                    const uint32_t first = f + H.c2_ld(b);
-                    const uint32_t last = first + H.c1_ld(b) - 1;
+                    const uint32_t first = f + (b == 0 ? 0u : H.hits_ld(b - 1));
+                    const uint32_t last = f + H.hits_ld(b) - 1;
                     H.cend(P) = (ix_t)last;
                     if (P == first) {
                         H.cfirst(last) = (ix_t)first;
@@ -842,13 +849,10 @@ struct Worker {
             }
             X.ncells += nfrag - 1;
             H.sync();
-            for (uint32_t b = (uint32_t)l; b < R; b += kWave) {
-                H.c1_st(b, 0);
-                H.c2_st(b, 0);
-            }
+            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.hits_st(b, 0);
             if (l == 0) {
-                H.sa(f) = 2;
-                H.sv(f) = (ix_t)c;
+                set_bit_atomic(H.split(), f);
+                set_bit_atomic(H.oldend(), c);
             }
             H.sync();
         }
@@ -870,13 +874,16 @@ struct Worker {
         // hash chain / active rules in position order (reference 253-468)
         for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            uint32_t f = 0, st = 0, x = 0;
+            // cells that did not split hash their common count (0 when
+            // untouched) plus their first position
+            uint32_t f = 0, x = 0;
+            bool sp = false;
             if (i < nns) {
                 f = H.sb(i);
-                st = H.sa(f);
-                x = (st == 1) ? (uint32_t)H.sv(f) + f : f;
+                sp = test_bit(H.split(), f);
+                x = (s > 1 && !sp) ? H.cnt_ld(H.elem(f)) + f : f;
             }
-            const uint64_t special = ballot(i < nns && st == 2);
+            const uint64_t special = ballot(sp);
             const uint32_t here = min((uint32_t)kWave, nns - base);
             if (s > 1 && special == 0) {
                 for (uint32_t t = 0; t < here; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
@@ -888,10 +895,10 @@ struct Worker {
                     continue;
                 }
                 const uint32_t fc = readlane(f, (int)t);
-                const uint32_t cold = H.sv(fc);
                 if (s == 1) {
+                    // two fragments: neighbours first
                     const uint32_t K = (uint32_t)H.cend(fc) - fc + 1;
-                    const uint32_t LL = cold - fc + 1;
+                    const uint32_t LL = (uint32_t)H.cend(fc + K) - fc + 1;
                     longcode = GCL_MASH(longcode, fc + K - 1);
                     if (test_bit(H.active(), fc) || K >= LL - K) {
                         add_active(X, fc + K);
@@ -902,8 +909,9 @@ struct Worker {
                     }
                     continue;
                 }
+                // fragments of the old cell, up to the one ending at its old last
                 uint32_t max_size = 0, max_first = fc;
-                for (uint32_t q = fc; q <= cold;) {
+                for (uint32_t q = fc;;) {
                     const uint32_t last = H.cend(q);
                     const uint32_t size = last - q + 1;
                     longcode = GCL_MASH(longcode, H.cnt_ld(H.elem(q)) + q);
@@ -911,13 +919,15 @@ struct Worker {
                         max_size = size;
                         max_first = q;
                     }
+                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
                     q = last + 1;
                 }
                 const bool was_active = test_bit(H.active(), fc);
-                for (uint32_t q = (uint32_t)H.cend(fc) + 1; q <= cold;) {
+                for (uint32_t q = (uint32_t)H.cend(fc) + 1;;) {
                     const uint32_t last = H.cend(q);
                     add_active(X, q);
                     if (last == q) hint = (int)q;
+                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
                     q = last + 1;
                 }
                 if (!was_active) {
@@ -927,30 +937,39 @@ struct Worker {
             }
         }
         H.sync();
-        // cell masks of the new fragments; reset the per-cell states.  The
-        // states are keyed by old first positions: walk the ns list again.
+        // cell masks of the new fragments; clear the split marks.  The
+        // marks are keyed by old first positions: walk the ns list again.
         for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             if (i < nns) {
                 const uint32_t f = H.sb(i);
-                const uint32_t st = H.sa(f);
-                if (st == 2) {
-                    const uint32_t cold = H.sv(f);
-                    for (uint32_t q = f; q <= cold;) {
+                if (test_bit(H.split(), f)) {
+                    for (uint32_t q = f;;) {
                         const uint32_t last = H.cend(q);
                         if (last > q) set_bit_atomic(H.nsmask(), q);
                         else clear_bit_atomic(H.nsmask(), q);
+                        if (test_bit(H.oldend(), last)) {
+                            clear_bit_atomic(H.oldend(), last);
+                            break;
+                        }
+                        if (last + 1 >= X.n) break;
                         q = last + 1;
                     }
+                    clear_bit_atomic(H.split(), f);
                 }
-                if (st) H.sa(f) = 0;
             }
         }
         H.sync();
+        // counts back to zero; the touched lists back to zero as well, since
+        // their words are the singleton step's counters
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            if (i < ntv) H.cnt_st(H.tv(i), 0);
+            if (i < ntv) {
+                H.cnt_st(H.tv(i), 0);
+                H.tv(i) = 0;
+            }
         }
+        for (uint32_t i = (uint32_t)l; i < ntc; i += kWave) H.tcl(i) = 0;
         H.sync();
     }
 
@@ -1724,12 +1743,13 @@ struct Worker {
             H.hits_st(p, 0);
             H.c1_st(p, 0);
             H.c2_st(p, 0);
-            H.sa(p) = 0;
         }
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) {
             H.lasts()[w] = rl[w];
             H.active()[w] = 0;
             H.vmask()[w] = 0;
+            H.split()[w] = 0;
+            H.oldend()[w] = 0;
         }
         nactive = 0;
         H.sync();
@@ -1917,7 +1937,9 @@ __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
     bind_scratch(W, P, base, L);
     W.H.h16 = (ix_t*)(base + L.hot16);
     W.H.h32 = (uint32_t*)(base + L.hot32);
-    W.H.n1 = (uint64_t)P.maxn + 2;
+    W.H.h64 = (uint64_t*)(base + L.hot64);
+    W.H.n1 = hot_stride(P.maxn);
+    W.H.nw1 = (uint64_t)P.maxn / 64 + 2;
     worker_loop(W, P);
 }
 
