@@ -194,6 +194,24 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x) {  // inclusive
     return x;
 }
 __device__ __forceinline__ uint32_t readlane(uint32_t v, int i) { return __builtin_amdgcn_readlane(v, i); }
+// Values that are the same in every lane (loaded from LDS, from the worker
+// object, or reduced over the wave) are moved to scalar registers, so that
+// the arithmetic and branches on them are scalar instructions.
+// (the builtin returns a signed int: each half is taken as uint32_t before
+// it is widened, so that the low half is not sign-extended)
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int32_t uni(int32_t x) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)x); }
+__device__ __forceinline__ uint64_t uni(uint64_t x) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
+}
+__device__ __forceinline__ int64_t uni(int64_t x) { return (int64_t)uni((uint64_t)x); }
+__device__ __forceinline__ bool uni(bool x) { return __builtin_amdgcn_readfirstlane((uint32_t)x) != 0u; }
+template <class T>
+__device__ __forceinline__ T* uni(T* ptr) {
+    return (T*)uni((uint64_t)ptr);
+}
 // polling load: no cache invalidation per poll; pair with agent_acquire()
 // once the awaited value has been seen
 __device__ __forceinline__ uint32_t agent_peek(const uint32_t* p) {
@@ -210,15 +228,23 @@ __device__ __forceinline__ bool agent_cas(uint32_t* p, uint32_t expect, uint32_t
 
 // ----------------------------------------------------------- hot storage
 
+// Load through a global-memory pointer (the pointers kept in the LDS worker
+// object lose their address space, which would make every load a flat one).
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // 16-bit counter arrays stored as pairs in 32-bit words: loads and stores are
 // 16-bit, increments and compare-and-swap go through the containing word
-// (values stay below 2^16 since every count is at most n <= 65535).
+// (values stay below 2^16 since every count is at most n <= 65535).  Loads
+// and stores are plain: wave_sync orders them against the other lanes.
 #define GCL_PACKED16(NAME, BASE)                                                                  \
     __device__ __forceinline__ uint32_t NAME##_ld(uint32_t i) {                                   \
-        return ((volatile uint16_t*)(BASE))[i];                                                   \
+        return ((const uint16_t*)(BASE))[i];                                                      \
     }                                                                                             \
     __device__ __forceinline__ void NAME##_st(uint32_t i, uint32_t v) {                           \
-        ((volatile uint16_t*)(BASE))[i] = (uint16_t)v;                                            \
+        ((uint16_t*)(BASE))[i] = (uint16_t)v;                                                     \
     }                                                                                             \
     __device__ __forceinline__ uint32_t NAME##_add(uint32_t i) {                                  \
         const uint32_t sh = (i & 1u) * 16u;                                                       \
@@ -227,7 +253,7 @@ __device__ __forceinline__ bool agent_cas(uint32_t* p, uint32_t expect, uint32_t
     __device__ __forceinline__ bool NAME##_cas(uint32_t i, uint32_t expect, uint32_t desired) {   \
         const uint32_t sh = (i & 1u) * 16u;                                                       \
         uint32_t* w = &(BASE)[i >> 1];                                                            \
-        uint32_t old = *(volatile uint32_t*)w;                                                    \
+        uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);      \
         for (int k = 0; k < 64; ++k) {                                                            \
             if (((old >> sh) & 0xFFFFu) != expect) return false;                                  \
             const uint32_t nw_ = (old & ~(0xFFFFu << sh)) | (desired << sh);                      \
@@ -258,6 +284,7 @@ struct LdsHot {
     uint64_t split[NW], oldend[NW];  // per general step: split cells (by first), their old lasts
     uint32_t ctr[4];
     const int64_t* rowp;
+    int64_t rbase;  // rowp[0]
 };
 
 // Hot state in LDS: addresses are compile-time constants.
@@ -290,15 +317,16 @@ struct HotL {
     __device__ __forceinline__ uint32_t* ctr() { return S().ctr; }
     __device__ __forceinline__ uint32_t row(uint32_t v) {
         if (kRowLds) return S().rowl[v];
-        const int64_t* rp = S().rowp;
-        return (uint32_t)(rp[v] - rp[0]);
+        return (uint32_t)(gload(S().rowp + v) - S().rbase);
     }
+    __device__ __forceinline__ const int64_t* rows() { return S().rowp; }
     __device__ __forceinline__ void sync() { wave_sync(); }
     __device__ __forceinline__ void bind(const int64_t* rowp, uint32_t n, uint8_t*, uint32_t) {
         if (kRowLds) {
             for (uint32_t i = (uint32_t)lane_id(); i <= n; i += kWave) S().rowl[i] = (uint32_t)(rowp[i] - rowp[0]);
         } else {
             S().rowp = rowp;
+            S().rbase = rowp[0];
         }
     }
 };
@@ -340,7 +368,9 @@ struct HotG {
     __device__ __forceinline__ uint64_t* vmask() { return B().vmask; }
     __device__ __forceinline__ uint64_t* lasts() { return B().lasts; }
     __device__ __forceinline__ uint32_t* ctr() { return B().ctr; }
-    __device__ __forceinline__ uint32_t row(uint32_t v) { return (uint32_t)(rowp[v] - rowp[0]); }
+    static constexpr bool kRowLds = false;
+    __device__ __forceinline__ uint32_t row(uint32_t v) { return (uint32_t)(gload(rowp + v) - gload(rowp)); }
+    __device__ __forceinline__ const int64_t* rows() { return rowp; }
     __device__ __forceinline__ void sync() { group_sync(); }
     __device__ __forceinline__ void bind(const int64_t* rp, uint32_t, uint8_t* base, uint32_t maxn) {
         rowp = rp;
@@ -356,6 +386,8 @@ struct Ctx {
     uint32_t n, nw, ncells, nactive;
     uint64_t steps, csr_bytes;
     const int32_t* col;
+    const int64_t* rp;  // global row offsets and rp[0] (rows outside LDS)
+    int64_t rb;
 };
 
 // ------------------------------------------------------------------ worker
@@ -408,14 +440,20 @@ struct Worker {
             else m[p >> 6] &= ~(1ull << (p & 63));
         }
     }
+    // row offset of v (local to the graph's col) inside a refinement
+    __device__ __forceinline__ uint32_t xrow(const Ctx& X, uint32_t v) {
+        if constexpr (Hot::kRowLds) return H.row(v);
+        else return (uint32_t)(gload(X.rp + v) - X.rb);
+    }
+    // p is wave-uniform
     __device__ __forceinline__ void add_active(Ctx& X, uint32_t p) {
-        if (p < X.n && !test_bit(H.active(), p)) {
+        if (p < X.n && !uni(test_bit(H.active(), p))) {
             set_bit_u(H.active(), p, true);
             ++X.nactive;
         }
     }
     __device__ __forceinline__ void remove_active(Ctx& X, uint32_t p) {
-        if (p < X.n && test_bit(H.active(), p)) {
+        if (p < X.n && uni(test_bit(H.active(), p))) {
             set_bit_u(H.active(), p, false);
             --X.nactive;
         }
@@ -433,7 +471,7 @@ struct Worker {
                 const uint32_t lo = readlane((uint32_t)bits, fl);
                 const uint32_t hi = readlane((uint32_t)(bits >> 32), fl);
                 const uint64_t word = ((uint64_t)hi << 32) | lo;
-                return (int32_t)((base + (uint32_t)fl) * 64u + (uint32_t)__builtin_ctzll(word));
+                return uni((int32_t)((base + (uint32_t)fl) * 64u + (uint32_t)__builtin_ctzll(word)));
             }
         }
         return -1;
@@ -546,15 +584,14 @@ struct Worker {
     // Refinement step, splitter {w} with deg(w) <= 64 (reference
     // src/partition_refinement.cpp:253-468, splitter.length == 1): cells
     // holding neighbours split into (neighbours, others).
-    __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t w, uint32_t& longcode, int& hint) {
+    __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t w, uint32_t e0, uint32_t deg, uint32_t& longcode,
+                                                   int& hint) {
         const int l = lane_id();
-        const uint32_t e0 = H.row(w);
-        const uint32_t deg = H.row(w + 1) - e0;
         X.csr_bytes += 12 + 4ull * deg;
         uint32_t u = 0, c = 0, f = 0, r = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
-            u = (uint32_t)X.col[e0 + (uint32_t)l];
+            u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
             c = H.cend(H.pos(u));
             f = H.cfirst(c);
             ns = f != c;
@@ -726,10 +763,10 @@ struct Worker {
             const uint32_t i = base + (uint32_t)l;
             if (i < s) {
                 const uint32_t w = H.elem(p + i);
-                const uint32_t a = H.row(w), b = H.row(w + 1);
+                const uint32_t a = xrow(X, w), b = xrow(X, w + 1);
                 degsum += b - a;
                 for (uint32_t x = a; x < b; ++x) {
-                    const uint32_t u = (uint32_t)X.col[x];
+                    const uint32_t u = (uint32_t)gload(X.col + x);
                     if (H.cnt_add(u) == 0u) {
                         const uint32_t idx = atomicAdd(&ctr[0], 1u);
                         H.tv(idx) = (ix_t)u;
@@ -737,9 +774,9 @@ struct Worker {
                 }
             }
         }
-        X.csr_bytes += 12ull * s + 4ull * wave_sum(degsum);
+        X.csr_bytes += 12ull * s + 4ull * uni(wave_sum(degsum));
         H.sync();
-        const uint32_t ntv = ctr[0];
+        const uint32_t ntv = uni(ctr[0]);
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             if (i < ntv) {
@@ -754,7 +791,7 @@ struct Worker {
             }
         }
         H.sync();
-        const uint32_t ntc = ctr[1];
+        const uint32_t ntc = uni(ctr[1]);
         // small cells: one lane each; large ones are compacted to the front
         // of the touched list (entry nbig + rank <= i was already read)
         uint32_t nbig = 0, nfrag_extra = 0;
@@ -776,13 +813,13 @@ struct Worker {
             if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
             nbig += (uint32_t)__popcll(bm);
         }
-        X.ncells += wave_sum(nfrag_extra);
+        X.ncells += uni(wave_sum(nfrag_extra));
         H.sync();
         // hits is all zero from here on: it holds the buckets of the
         // counting sort below (indices 0..R-1, cleared after each cell)
         for (uint32_t j = 0; j < nbig; ++j) {
-            const uint32_t c = H.tcl(j);
-            const uint32_t f = H.cfirst(c);
+            const uint32_t c = uni((uint32_t)H.tcl(j));
+            const uint32_t f = uni((uint32_t)H.cfirst(c));
             const uint32_t L = c - f + 1;
             uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
             for (uint32_t base = 0; base < L; base += kWave) {
@@ -794,8 +831,8 @@ struct Worker {
                     kmax = max(kmax, kv);
                 }
             }
-            kmin = wave_min(kmin);
-            kmax = wave_max(kmax);
+            kmin = uni(wave_min(kmin));
+            kmax = uni(wave_max(kmax));
             if (kmin == kmax) continue;  // uniform: nothing moves
             const uint32_t R = kmax - kmin + 1;
             for (uint32_t base = 0; base < L; base += kWave) {
@@ -897,10 +934,10 @@ struct Worker {
                 const uint32_t fc = readlane(f, (int)t);
                 if (s == 1) {
                     // two fragments: neighbours first
-                    const uint32_t K = (uint32_t)H.cend(fc) - fc + 1;
-                    const uint32_t LL = (uint32_t)H.cend(fc + K) - fc + 1;
+                    const uint32_t K = uni((uint32_t)H.cend(fc)) - fc + 1;
+                    const uint32_t LL = uni((uint32_t)H.cend(fc + K)) - fc + 1;
                     longcode = GCL_MASH(longcode, fc + K - 1);
-                    if (test_bit(H.active(), fc) || K >= LL - K) {
+                    if (uni(test_bit(H.active(), fc)) || K >= LL - K) {
                         add_active(X, fc + K);
                         if (LL - K == 1) hint = (int)(fc + K);
                     } else {
@@ -912,22 +949,22 @@ struct Worker {
                 // fragments of the old cell, up to the one ending at its old last
                 uint32_t max_size = 0, max_first = fc;
                 for (uint32_t q = fc;;) {
-                    const uint32_t last = H.cend(q);
+                    const uint32_t last = uni((uint32_t)H.cend(q));
                     const uint32_t size = last - q + 1;
-                    longcode = GCL_MASH(longcode, H.cnt_ld(H.elem(q)) + q);
+                    longcode = GCL_MASH(longcode, uni(H.cnt_ld(H.elem(q))) + q);
                     if (size > max_size) {
                         max_size = size;
                         max_first = q;
                     }
-                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                    if (uni(test_bit(H.oldend(), last)) || last + 1 >= X.n) break;
                     q = last + 1;
                 }
-                const bool was_active = test_bit(H.active(), fc);
-                for (uint32_t q = (uint32_t)H.cend(fc) + 1;;) {
-                    const uint32_t last = H.cend(q);
+                const bool was_active = uni(test_bit(H.active(), fc));
+                for (uint32_t q = uni((uint32_t)H.cend(fc)) + 1;;) {
+                    const uint32_t last = uni((uint32_t)H.cend(q));
                     add_active(X, q);
                     if (last == q) hint = (int)q;
-                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                    if (uni(test_bit(H.oldend(), last)) || last + 1 >= X.n) break;
                     q = last + 1;
                 }
                 if (!was_active) {
@@ -977,13 +1014,16 @@ struct Worker {
     __device__ int refine(int32_t init_active) {
         const int l = lane_id();
         Ctx X;
-        X.n = n;
-        X.nw = nw;
-        X.ncells = ncells;
-        X.nactive = nactive;
-        X.steps = steps;
-        X.csr_bytes = csr_bytes;
-        X.col = col;
+        X.n = uni(n);
+        X.nw = uni(nw);
+        X.ncells = uni(ncells);
+        X.nactive = uni(nactive);
+        X.steps = uni(steps);
+        X.csr_bytes = uni(csr_bytes);
+        X.col = uni(col);
+        X.rp = uni(H.rows());
+        X.rb = Hot::kRowLds ? 0 : uni(gload(X.rp));
+        init_active = uni(init_active);
         if (init_active < 0) {
             uint32_t total = 0;
             for (uint32_t base = 0; base < X.n; base += kWave) {
@@ -1000,7 +1040,7 @@ struct Worker {
         H.sync();
         uint32_t longcode = X.ncells;
         int hint = 0;
-        uint64_t bud = budget;
+        uint64_t bud = uni(budget);
         while (X.ncells < X.n && X.nactive > 0) {
             if (bud == 0) {
                 status = GCL_E_INTERNAL;  // step budget exhausted
@@ -1009,17 +1049,19 @@ struct Worker {
             --bud;
             int32_t sp = first_set_from(H.active(), (uint32_t)hint, X.nw);
             if (sp < 0) sp = first_set_from(H.active(), 0, X.nw);
-            const uint32_t p = (uint32_t)sp;
+            const uint32_t p = uni((uint32_t)sp);
             set_bit_u(H.active(), p, false);
             --X.nactive;
             ++X.steps;
-            const uint32_t e = H.cend(p);
+            const uint32_t e = uni((uint32_t)H.cend(p));
             longcode = GCL_MASH(longcode, p + e);
             if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
             H.sync();
             if (e == p) {
-                const uint32_t w = H.elem(p);
-                if (H.row(w + 1) - H.row(w) <= (uint32_t)kWave) singleton_step(X, w, longcode, hint);
+                const uint32_t w = uni((uint32_t)H.elem(p));
+                const uint32_t e0 = uni(xrow(X, w));
+                const uint32_t deg = uni(xrow(X, w + 1)) - e0;
+                if (deg <= (uint32_t)kWave) singleton_step(X, w, e0, deg, longcode, hint);
                 else general_step(X, p, e, longcode, hint);
             } else {
                 general_step(X, p, e, longcode, hint);
@@ -1196,6 +1238,7 @@ struct Worker {
     // (src/CSRGraph.cpp:332-351).
     __device__ void store_cert(uint32_t* keys, uint32_t* row, ix_t* label) {
         const int l = lane_id();
+        const int32_t* cl = uni(col);
         uint32_t run = 0;
         for (uint32_t base = 0; base < n; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
@@ -1212,7 +1255,7 @@ struct Worker {
                 const uint32_t a = H.row(v);
                 uint32_t kk[8];
 #pragma unroll
-                for (uint32_t t = 0; t < 8; ++t) kk[t] = t < d ? i * n + H.pos((uint32_t)col[a + t]) : 0xFFFFFFFFu;
+                for (uint32_t t = 0; t < 8; ++t) kk[t] = t < d ? i * n + H.pos((uint32_t)gload(cl + a + t)) : 0xFFFFFFFFu;
 #pragma unroll
                 for (uint32_t k = 2; k <= 8; k <<= 1)
 #pragma unroll
@@ -1237,7 +1280,7 @@ struct Worker {
                 label[i] = (ix_t)v;
                 const uint32_t a = H.row(v);
                 for (uint32_t t = 0; t < d; ++t) {
-                    const uint32_t key = i * n + H.pos((uint32_t)col[a + t]);
+                    const uint32_t key = i * n + H.pos((uint32_t)gload(cl + a + t));
                     uint32_t z = t;
                     while (z > 0 && keys[o + z - 1] > key) {
                         keys[o + z] = keys[o + z - 1];
@@ -1258,6 +1301,9 @@ struct Worker {
     // smallest element of the symmetric difference is the greater one.
     __device__ int compare_cert(const uint32_t* keys, const uint32_t* row) {
         const int l = lane_id();
+        const int32_t* cl = uni(col);
+        keys = uni(keys);
+        row = uni(row);
         for (uint32_t base = 0; base < n; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             bool differs = false;
@@ -1265,15 +1311,15 @@ struct Worker {
                 const uint32_t v = H.elem(i);
                 const uint32_t a = H.row(v);
                 const uint32_t d = H.row(v + 1) - a;
-                const uint32_t r0 = row[i], r1 = row[i + 1];
+                const uint32_t r0 = gload(row + i), r1 = gload(row + i + 1);
                 if (r1 - r0 != d) {
                     differs = true;
                 } else if (d <= 8) {
                     uint32_t ref[8], cur[8];
 #pragma unroll
                     for (uint32_t t = 0; t < 8; ++t) {
-                        ref[t] = t < d ? keys[r0 + t] : 0xFFFFFFFFu;
-                        cur[t] = t < d ? (uint32_t)col[a + t] : 0u;
+                        ref[t] = t < d ? gload(keys + r0 + t) : 0xFFFFFFFFu;
+                        cur[t] = t < d ? (uint32_t)gload(cl + a + t) : 0u;
                     }
 #pragma unroll
                     for (uint32_t t = 0; t < 8; ++t) {
@@ -1287,14 +1333,14 @@ struct Worker {
                     }
                 } else {
                     for (uint32_t t = 0; t < d && !differs; ++t) {
-                        const uint32_t key = i * n + H.pos((uint32_t)col[a + t]);
+                        const uint32_t key = i * n + H.pos((uint32_t)gload(cl + a + t));
                         uint32_t lo = r0, hi = r1;
                         while (lo < hi) {
                             const uint32_t mid = (lo + hi) >> 1;
-                            if (keys[mid] < key) lo = mid + 1;
+                            if (gload(keys + mid) < key) lo = mid + 1;
                             else hi = mid;
                         }
-                        if (lo == r1 || keys[lo] != key) differs = true;
+                        if (lo == r1 || gload(keys + lo) != key) differs = true;
                     }
                 }
             }
@@ -1304,23 +1350,23 @@ struct Worker {
                 const uint32_t v = H.elem(i0);
                 const uint32_t a = H.row(v);
                 const uint32_t d = H.row(v + 1) - a;
-                const uint32_t r0 = row[i0], r1 = row[i0 + 1];
+                const uint32_t r0 = gload(row + i0), r1 = gload(row + i0 + 1);
                 uint32_t min_cur = 0xFFFFFFFFu, min_ref = 0xFFFFFFFFu;
                 for (uint32_t t = (uint32_t)l; t < d; t += kWave) {
-                    const uint32_t key = i0 * n + H.pos((uint32_t)col[a + t]);
+                    const uint32_t key = i0 * n + H.pos((uint32_t)gload(cl + a + t));
                     bool found = false;
                     for (uint32_t z = r0; z < r1; ++z)
-                        if (keys[z] == key) {
+                        if (gload(keys + z) == key) {
                             found = true;
                             break;
                         }
                     if (!found) min_cur = min(min_cur, key);
                 }
                 for (uint32_t z = r0 + (uint32_t)l; z < r1; z += kWave) {
-                    const uint32_t key = keys[z];
+                    const uint32_t key = gload(keys + z);
                     bool found = false;
                     for (uint32_t t = 0; t < d; ++t)
-                        if (i0 * n + H.pos((uint32_t)col[a + t]) == key) {
+                        if (i0 * n + H.pos((uint32_t)gload(cl + a + t)) == key) {
                             found = true;
                             break;
                         }
