@@ -173,27 +173,34 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+__device__ __forceinline__ uint32_t readlane(uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane(v, i); }
+// Wave-wide scans with DPP lane moves (no LDS round trips): shifts by 1, 2,
+// 4 and 8 inside each row of 16 lanes, then row sums carried into the later
+// rows with row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3).  Every
+// lane of the wave must be active.
+#define GCL_DPP(x, ctrl, rows) ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)(x), ctrl, rows, 0xF, false))
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {  // inclusive sum
+    x += GCL_DPP(x, 0x111, 0xF);
+    x += GCL_DPP(x, 0x112, 0xF);
+    x += GCL_DPP(x, 0x114, 0xF);
+    x += GCL_DPP(x, 0x118, 0xF);
+    x += GCL_DPP(x, 0x142, 0xA);
+    x += GCL_DPP(x, 0x143, 0xC);
     return x;
 }
-__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-    for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, kWave));
+__device__ __forceinline__ uint32_t wave_scan_max(uint32_t x) {  // inclusive max
+    x = max(x, GCL_DPP(x, 0x111, 0xF));
+    x = max(x, GCL_DPP(x, 0x112, 0xF));
+    x = max(x, GCL_DPP(x, 0x114, 0xF));
+    x = max(x, GCL_DPP(x, 0x118, 0xF));
+    x = max(x, GCL_DPP(x, 0x142, 0xA));
+    x = max(x, GCL_DPP(x, 0x143, 0xC));
     return x;
 }
-__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
-    for (int o = 32; o > 0; o >>= 1) x = max(x, (uint32_t)__shfl_xor(x, o, kWave));
-    return x;
-}
-__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {  // inclusive
-    const int l = lane_id();
-    for (int o = 1; o < kWave; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, kWave);
-        if (l >= o) x += y;
-    }
-    return x;
-}
-__device__ __forceinline__ uint32_t readlane(uint32_t v, int i) { return __builtin_amdgcn_readlane(v, i); }
+// reductions: the result is wave-uniform (a scalar register)
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) { return readlane(wave_scan(x), kWave - 1); }
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) { return readlane(wave_scan_max(x), kWave - 1); }
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) { return ~wave_max(~x); }
 // Values that are the same in every lane (loaded from LDS, from the worker
 // object, or reduced over the wave) are moved to scalar registers, so that
 // the arithmetic and branches on them are scalar instructions.
@@ -752,46 +759,78 @@ struct Worker {
     __device__ void general_step(Ctx& X, uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         const uint32_t s = e - p + 1;
-        uint32_t* ctr = H.ctr();
-        if (l == 0) {
-            ctr[0] = 0;
-            ctr[1] = 0;
-        }
-        H.sync();
-        uint32_t degsum = 0;
+        // counts: cnt[u] = neighbours of u in the splitter; the first touch of
+        // a vertex appends it to the tv list (ranks from ballots)
+        uint32_t ntv = 0;
+        uint64_t degsum = 0;
         for (uint32_t base = 0; base < s; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
+            uint32_t a = 0, d = 0;
             if (i < s) {
                 const uint32_t w = H.elem(p + i);
-                const uint32_t a = xrow(X, w), b = xrow(X, w + 1);
-                degsum += b - a;
-                for (uint32_t x = a; x < b; ++x) {
-                    const uint32_t u = (uint32_t)gload(X.col + x);
-                    if (H.cnt_add(u) == 0u) {
-                        const uint32_t idx = atomicAdd(&ctr[0], 1u);
-                        H.tv(idx) = (ix_t)u;
+                a = xrow(X, w);
+                d = xrow(X, w + 1) - a;
+            }
+            const uint32_t incl = wave_scan(d);
+            const uint32_t total = readlane(incl, kWave - 1);
+            degsum += total;
+            const uint32_t nv = min(s - base, (uint32_t)kWave);
+            if (total <= (uint32_t)kWave) {
+                // one (vertex, neighbour) pair per lane: lane t takes the
+                // first vertex whose degree run [incl - d, incl) ends after t
+                const uint32_t off = a + d - incl;  // col index of a pair = t + off of its vertex
+                uint32_t x = 0;
+                bool found = false;
+                for (uint32_t j = 0; j < nv; ++j) {
+                    const uint32_t ij = readlane(incl, (int)j);
+                    const uint32_t oj = readlane(off, (int)j);
+                    if (!found && (uint32_t)l < ij) {
+                        found = true;
+                        x = (uint32_t)l + oj;
                     }
+                }
+                bool first = false;
+                uint32_t u = 0;
+                if ((uint32_t)l < total) {
+                    u = (uint32_t)gload(X.col + x);
+                    first = H.cnt_add(u) == 0u;
+                }
+                const uint64_t fm = ballot(first);
+                if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
+                ntv += (uint32_t)__popcll(fm);
+            } else {
+                const uint32_t dmax = wave_max(d);
+                for (uint32_t t = 0; t < dmax; ++t) {
+                    bool first = false;
+                    uint32_t u = 0;
+                    if (t < d) {
+                        u = (uint32_t)gload(X.col + a + t);
+                        first = H.cnt_add(u) == 0u;
+                    }
+                    const uint64_t fm = ballot(first);
+                    if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
+                    ntv += (uint32_t)__popcll(fm);
                 }
             }
         }
-        X.csr_bytes += 12ull * s + 4ull * uni(wave_sum(degsum));
+        X.csr_bytes += 12ull * s + 4ull * degsum;
         H.sync();
-        const uint32_t ntv = uni(ctr[0]);
+        // touched non-singleton cells, each once (hits[c] counts its touches)
+        uint32_t ntc = 0;
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
+            bool firstc = false;
+            uint32_t c = 0;
             if (i < ntv) {
                 const uint32_t u = H.tv(i);
-                const uint32_t c = H.cend(H.pos(u));
-                if (H.cfirst(c) != c) {
-                    if (H.hits_add(c) == 0u) {
-                        const uint32_t idx = atomicAdd(&ctr[1], 1u);
-                        H.tcl(idx) = (ix_t)c;
-                    }
-                }
+                c = H.cend(H.pos(u));
+                if (H.cfirst(c) != c) firstc = H.hits_add(c) == 0u;
             }
+            const uint64_t cm = ballot(firstc);
+            if (firstc) H.tcl(ntc + lanes_below(cm)) = (ix_t)c;
+            ntc += (uint32_t)__popcll(cm);
         }
         H.sync();
-        const uint32_t ntc = uni(ctr[1]);
         // small cells: one lane each; large ones are compacted to the front
         // of the touched list (entry nbig + rank <= i was already read)
         uint32_t nbig = 0, nfrag_extra = 0;
