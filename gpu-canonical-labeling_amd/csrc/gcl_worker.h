@@ -693,11 +693,12 @@ struct Worker {
     // number of fragments; a cell that splits is marked for the hash pass
     // (bit f of split, bit c of oldend).
     static constexpr uint32_t kSmallCell = 8;
+    template <uint32_t N>  // network size: a power of two >= the cell size
     __device__ __forceinline__ uint32_t regroup_small(uint32_t f, uint32_t c, uint32_t s) {
         const uint32_t L = c - f + 1;
-        uint32_t kk[kSmallCell];
+        uint32_t kk[N];
 #pragma unroll
-        for (uint32_t t = 0; t < kSmallCell; ++t) {
+        for (uint32_t t = 0; t < N; ++t) {
             if (t < L) {
                 const uint32_t v = H.elem(f + t);
                 const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
@@ -706,35 +707,34 @@ struct Worker {
                 kk[t] = 0xFFFFFFFFu;
             }
         }
+        // bitonic network; each compare-exchange is a min and a max
 #pragma unroll
-        for (uint32_t k = 2; k <= kSmallCell; k <<= 1) {
+        for (uint32_t k = 2; k <= N; k <<= 1) {
 #pragma unroll
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
 #pragma unroll
-                for (uint32_t i = 0; i < kSmallCell; ++i) {
+                for (uint32_t i = 0; i < N; ++i) {
                     const uint32_t ixj = i ^ j;
                     if (ixj > i) {
+                        const uint32_t lo = min(kk[i], kk[ixj]), hi = max(kk[i], kk[ixj]);
                         const bool up = (i & k) == 0;
-                        const uint32_t a = kk[i], b = kk[ixj];
-                        if ((a > b) == up) {
-                            kk[i] = b;
-                            kk[ixj] = a;
-                        }
+                        kk[i] = up ? lo : hi;
+                        kk[ixj] = up ? hi : lo;
                     }
                 }
             }
         }
         uint32_t kmin = kk[0] >> 16, kmax = 0;
 #pragma unroll
-        for (uint32_t t = 0; t < kSmallCell; ++t)
+        for (uint32_t t = 0; t < N; ++t)
             if (t + 1 == L) kmax = kk[t] >> 16;
         if (kmin == kmax) return 1;
         uint32_t nfrag = 0, end = 0;
 #pragma unroll
-        for (int t = (int)kSmallCell - 1; t >= 0; --t) {
+        for (int t = (int)N - 1; t >= 0; --t) {
             if ((uint32_t)t < L) {
                 const uint32_t key = kk[t] >> 16, v = kk[t] & 0xFFFFu;
-                if ((uint32_t)t + 1 == L || (kk[t + 1 < (int)kSmallCell ? t + 1 : t] >> 16) != key) end = (uint32_t)t;
+                if ((uint32_t)t + 1 == L || (kk[t + 1 < (int)N ? t + 1 : t] >> 16) != key) end = (uint32_t)t;
                 H.elem(f + (uint32_t)t) = (ix_t)v;
                 H.pos(v) = (ix_t)(f + (uint32_t)t);
                 H.cend(f + (uint32_t)t) = (ix_t)(f + end);
@@ -836,17 +836,20 @@ struct Worker {
         uint32_t nbig = 0, nfrag_extra = 0;
         for (uint32_t base = 0; base < ntc; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            bool big = false;
-            uint32_t c = 0;
+            bool big = false, small = false;
+            uint32_t c = 0, f = 0;
             if (i < ntc) {
                 c = H.tcl(i);
-                const uint32_t f = H.cfirst(c);
+                f = H.cfirst(c);
                 H.hits_st(c, 0);
-                if (c - f + 1 <= kSmallCell) {
-                    nfrag_extra += regroup_small(f, c, s) - 1;
-                } else {
-                    big = true;
-                }
+                small = c - f + 1 <= kSmallCell;
+                big = !small;
+            }
+            // the 4-wide network when no small cell here has more than 4 members
+            if (ballot(small && c - f + 1 > 4)) {
+                if (small) nfrag_extra += regroup_small<8>(f, c, s) - 1;
+            } else {
+                if (small) nfrag_extra += regroup_small<4>(f, c, s) - 1;
             }
             const uint64_t bm = ballot(big);
             if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
@@ -961,15 +964,21 @@ struct Worker {
             }
             const uint64_t special = ballot(sp);
             const uint32_t here = min((uint32_t)kWave, nns - base);
-            if (s > 1 && special == 0) {
-                for (uint32_t t = 0; t < here; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
-                continue;
-            }
             for (uint32_t t = 0; t < here; ++t) {
-                if (!((special >> t) & 1ull)) {
-                    if (s > 1) longcode = GCL_MASH(longcode, readlane(x, (int)t));
-                    continue;
+                // run of cells that did not split, up to the next split one
+                const uint64_t rest = special >> t;
+                const uint32_t nxt = rest ? t + (uint32_t)__builtin_ctzll(rest) : here;
+                if (s > 1) {
+                    for (; t + 4 <= nxt; t += 4) {
+                        longcode = GCL_MASH(longcode, readlane(x, (int)t));
+                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 1));
+                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 2));
+                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 3));
+                    }
+                    for (; t < nxt; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
                 }
+                t = nxt;
+                if (t >= here) break;
                 const uint32_t fc = readlane(f, (int)t);
                 if (s == 1) {
                     // two fragments: neighbours first
@@ -1086,8 +1095,17 @@ struct Worker {
                 break;
             }
             --bud;
-            int32_t sp = first_set_from(H.active(), (uint32_t)hint, X.nw);
-            if (sp < 0) sp = first_set_from(H.active(), 0, X.nw);
+            // splitter: the first active position at or after the hint, else
+            // the first one; the hint's own word is tested first (one scalar load)
+            const uint32_t hw = (uint32_t)hint >> 6;
+            const uint64_t near = uni(H.active()[hw]) & (~0ull << ((uint32_t)hint & 63u));
+            int32_t sp;
+            if (near) {
+                sp = (int32_t)(hw * 64u + (uint32_t)__builtin_ctzll(near));
+            } else {
+                sp = first_set_from(H.active(), (hw + 1) * 64u, X.nw);
+                if (sp < 0) sp = first_set_from(H.active(), 0, X.nw);
+            }
             const uint32_t p = uni((uint32_t)sp);
             set_bit_u(H.active(), p, false);
             --X.nactive;
