@@ -37,6 +37,7 @@ typedef uint16_t ix_t;
 constexpr int kWave = 64;
 constexpr uint32_t kMaxRecords = 4096;  // speculation records per launch (64 mask words)
 constexpr uint32_t kNoKey = 0u;
+constexpr uint32_t kFailedCode = 0xFFFFFFFFu;  // speculation item whose refinement failed
 
 struct LevelRec {
     uint32_t tc_first, tc_len, cursor, arena_off;
@@ -1668,7 +1669,7 @@ struct Worker {
 
     // Root child v: adopt a helper's refinement, or claim and refine it here.
     // Returns true when v was handled (partition at level 2, code set).
-    __device__ bool spec_take(uint32_t v) {
+    __device__ bool spec_take(const KParams& P, uint32_t v) {
         if (spec_slot < 0) return false;
         const ix_t* vert = (const ix_t*)(spec_rec + SL.vert);
         uint32_t* state = spec_state();
@@ -1684,25 +1685,54 @@ struct Worker {
         if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 3u) ? 1u : 0u;
         won = __shfl(won, 0, kWave);
         if (won) return false;  // withdrawn: refine it here
-        // a helper has it: wait until done, then copy its partition
+        // a helper has it: refine later children meanwhile, then copy its partition
         while (agent_peek(&state[j]) != (E | 2u)) {
             if (wall_clock64() > deadline) {
                 status = GCL_E_INTERNAL;  // a helper did not finish in time
                 return false;
             }
-            __builtin_amdgcn_s_sleep(2);
+            if (!spec_help_own(P, E)) __builtin_amdgcn_s_sleep(2);
         }
         agent_acquire();
         const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
         const uint64_t* il = (const uint64_t*)(spec_rec + SL.lasts) + (uint64_t)j * (P_nw1());
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = il[w];
-        code = (int)((const uint32_t*)(spec_rec + SL.code))[j];
+        const uint32_t hc = ((const uint32_t*)(spec_rec + SL.code))[j];
+        if (hc == kFailedCode) {
+            status = GCL_E_INTERNAL;  // the helper's refinement failed
+            return false;
+        }
+        code = (int)hc;
         steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
         csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
         H.sync();
         positions_from_elem();
         rebuild_from_lasts();
+        return true;
+    }
+    // While a helper refines the child this worker waits for, claim a later
+    // unclaimed child of its own record and refine it as a helper would.
+    // The adoption that follows replaces the partition; the counters that
+    // the adoption adds from the record are restored here, so nothing is
+    // counted twice.  Returns true when it refined one.
+    __device__ bool spec_help_own(const KParams& P, uint32_t E) {
+        SpecHeader* hd = spec_header();
+        uint32_t idx = 0xFFFFFFFFu;
+        if (lane_id() == 0 &&
+            __hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < spec_count) {
+            const uint32_t t = atomicAdd(&hd->next, 1u);
+            if (t < spec_count && agent_cas(&spec_state()[t], E | 0u, E | 1u)) idx = t;
+        }
+        idx = uni(readlane(idx, 0));
+        if (idx == 0xFFFFFFFFu) return false;
+        const uint64_t st0 = steps, by0 = csr_bytes, bud0 = budget;
+        const int32_t s0 = status;
+        help_item(P, spec_rec, idx, E);
+        steps = st0;
+        csr_bytes = by0;
+        budget = bud0;
+        status = s0;
         return true;
     }
     __device__ uint64_t P_n1() const { return (uint64_t)spec_maxn + 2; }
@@ -1770,7 +1800,7 @@ struct Worker {
             if (level == 1 && spec_pending) spec_publish(P, r, graph);
             const uint32_t v = arena_mem[r.arena_off + cur];
             lv[li].cursor = cur + 1;
-            if (level == 1 && spec_take(v)) {
+            if (level == 1 && spec_take(P, v)) {
                 ++nodes;
                 seq[seq_size] = (ix_t)v;
                 ++seq_size;
@@ -1869,7 +1899,9 @@ struct Worker {
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) oe[p] = H.elem(p);
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) ol[w] = H.lasts()[w];
         if (lane_id() == 0) {
-            ((uint32_t*)(rec + L.code))[j] = (uint32_t)code;
+            // a failed refinement is handed to the owner as a code no
+            // refinement produces (codes are below 2^15)
+            ((uint32_t*)(rec + L.code))[j] = status ? kFailedCode : (uint32_t)code;
             ((uint32_t*)(rec + L.ncells))[j] = ncells;
             ((uint64_t*)(rec + L.steps))[2 * j] = steps - s0;
             ((uint64_t*)(rec + L.steps))[2 * j + 1] = csr_bytes - b0;
