@@ -33,7 +33,7 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I" + INC, "-I" + CSRC, "-pthread"]
 
-HOST_SOURCES = ["gcl_api.cpp", "gcl_cpu.cpp", "gcl_graph.cpp", "gcl_workload.cpp"]
+HOST_SOURCES = ["gcl_api.cpp", "gcl_cpu.cpp", "gcl_graph.cpp", "gcl_server.cpp", "gcl_workload.cpp"]
 DEVICE_SOURCES = ["gcl_device.hip"]
 
 

@@ -34,6 +34,36 @@ gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32
 void device_batch_destroy(DeviceBatch* db);
 int32_t device_batch_size(const DeviceBatch* db);
 
+// Launch slot of the server: fixed device buffers and pinned staging.
+struct SlotConfig {
+    int32_t max_graphs = 0;       // graphs per launch
+    int64_t max_vertices = 0;     // sum of n per launch
+    int64_t max_nnz = 0;          // sum of directed edges per launch
+    int32_t max_n = 0;            // largest graph
+    int64_t max_graph_nnz = 0;    // directed edges of the largest graph
+    uint32_t max_aut = 1;
+    uint32_t ring_cap = 64;       // automorphism slots per worker
+    int32_t share = 2;            // launches meant to be resident at once
+    int32_t global_workers = 256; // workers of the n > 4096 class
+    uint64_t scratch_bytes_per_class = 2ull << 30;
+    uint64_t spec_bytes_per_class = 1ull << 30;
+};
+struct SlotStaging {  // pinned host arrays of a slot
+    int64_t* vtx;
+    int64_t* row;
+    int32_t* col;
+    int32_t* color;
+    int32_t* label;
+    DeviceGraphResult* out;
+};
+struct DeviceSlot;
+gcl_status slot_create(const SlotConfig& c, DeviceSlot** out);
+void slot_destroy(DeviceSlot* s);
+SlotStaging slot_staging(DeviceSlot* s);
+int slot_class_of(const DeviceSlot* s, int64_t n, uint64_t nnz);  // -1 when the graph does not fit
+gcl_status slot_launch(DeviceSlot* s, int32_t graphs, uint32_t max_aut, bool colored);  // staged batch
+gcl_status slot_poll(DeviceSlot* s, bool* finished);
+
 int32_t device_count();
 uint64_t step_budget();
 uint32_t spec_min_n();

@@ -461,6 +461,311 @@ struct DeviceBatch {
     }
 };
 
+// ------------------------------------------------------------- launch slot
+// A launch context with fixed buffers, for the server: every device buffer,
+// the per-class worker scratch and speculation records, and pinned host
+// staging are allocated once; a batch within the capacity is staged, copied
+// and launched asynchronously with no allocation.
+struct DeviceSlot {
+    struct Cls {
+        int cls = -1;
+        uint32_t maxn = 0;
+        uint64_t max_nnz = 0;
+        int64_t grid = 0;
+        uint32_t records = 0, spec_slots = 0;
+        uint64_t spec_stride = 0, scratch_stride = 0, arena = 0;
+        uint32_t levels = 0;
+        DevBuf scratch, spec, board;
+        size_t board_words = 0;
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        int32_t count = 0;    // graphs of the current launch
+        size_t item_off = 0;
+    };
+    SlotConfig cfg;
+    uint32_t ring = 1;
+    DevBuf vtx, row, col, label, out, items, ctr, ctr_init, color, init;
+    // pinned staging (host side of the copies)
+    int64_t* h_vtx = nullptr;
+    int64_t* h_row = nullptr;
+    int32_t* h_col = nullptr;
+    int32_t* h_color = nullptr;  // per vertex; used when a launch has coloured graphs
+    int32_t* h_init = nullptr;   // per graph: vertices in (colour, id) order
+    int32_t* h_items = nullptr;
+    int32_t* h_label = nullptr;
+    DeviceGraphResult* h_out = nullptr;
+    unsigned int* h_ctr_init = nullptr;
+    std::vector<Cls> classes;
+    std::vector<unsigned long long> board_init_h;
+    hipStream_t main = nullptr;
+    hipEvent_t start = nullptr, done = nullptr;
+    int32_t G = 0;
+    int64_t total_n = 0, total_nnz = 0;
+    bool busy = false;
+
+    ~DeviceSlot() {
+        if (main) (void)hipStreamSynchronize(main);
+        for (auto& c : classes) {
+            if (c.done) (void)hipEventDestroy(c.done);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+        }
+        if (start) (void)hipEventDestroy(start);
+        if (done) (void)hipEventDestroy(done);
+        if (main) (void)hipStreamDestroy(main);
+        for (void* p : {(void*)h_vtx, (void*)h_row, (void*)h_col, (void*)h_items, (void*)h_label, (void*)h_out,
+                        (void*)h_ctr_init, (void*)h_color, (void*)h_init})
+            if (p) (void)hipHostFree(p);
+    }
+
+    template <class T>
+    gcl_status host_alloc(T** p, size_t count) {
+        GCL_HIP(hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault));
+        return GCL_OK;
+    }
+
+    gcl_status create(const SlotConfig& c) {
+        cfg = c;
+        if (cfg.max_graphs <= 0 || cfg.max_vertices <= 0 || cfg.max_nnz < 0 || cfg.max_n <= 0 ||
+            cfg.max_n > kMaxVertices || cfg.max_aut == 0)
+            return fail(GCL_E_INVALID, "invalid slot configuration");
+        if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
+        GCL_HIP(hipSetDevice(0));
+        ring = std::max(1u, std::min(cfg.max_aut, cfg.ring_cap));
+        hipDeviceProp_t prop;
+        GCL_HIP(hipGetDeviceProperties(&prop, 0));
+        const size_t G1 = (size_t)cfg.max_graphs + 1, V1 = (size_t)cfg.max_vertices + 1;
+        GCL_HIP(hipMalloc(&vtx.p, sizeof(int64_t) * G1));
+        GCL_HIP(hipMalloc(&row.p, sizeof(int64_t) * V1));
+        GCL_HIP(hipMalloc(&col.p, sizeof(int32_t) * (size_t)std::max<int64_t>(cfg.max_nnz, 1)));
+        GCL_HIP(hipMalloc(&label.p, sizeof(int32_t) * V1));
+        GCL_HIP(hipMalloc(&out.p, sizeof(DeviceGraphResult) * G1));
+        GCL_HIP(hipMalloc(&items.p, sizeof(int32_t) * G1));
+        GCL_HIP(hipMalloc(&ctr.p, sizeof(unsigned int) * 64));
+        GCL_HIP(hipMalloc(&ctr_init.p, sizeof(unsigned int) * 64));
+        GCL_HIP(hipMalloc(&color.p, sizeof(int32_t) * V1));
+        GCL_HIP(hipMalloc(&init.p, sizeof(int32_t) * V1));
+        gcl_status st;
+        if ((st = host_alloc(&h_color, V1)) || (st = host_alloc(&h_init, V1))) return st;
+        if ((st = host_alloc(&h_vtx, G1)) || (st = host_alloc(&h_row, V1)) ||
+            (st = host_alloc(&h_col, (size_t)std::max<int64_t>(cfg.max_nnz, 1))) || (st = host_alloc(&h_items, G1)) ||
+            (st = host_alloc(&h_label, V1)) || (st = host_alloc(&h_out, G1)) || (st = host_alloc(&h_ctr_init, 64)))
+            return st;
+        GCL_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+        GCL_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+        GCL_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        // classes provisioned up to the largest graph accepted
+        int last = kClasses;
+        for (int k = 0; k < kClasses; ++k)
+            if ((uint32_t)cfg.max_n <= kClassN[k]) {
+                last = k;
+                break;
+            }
+        const uint32_t smin = spec_min_n();
+        for (int k = 0; k <= last; ++k) {
+            Cls C;
+            C.cls = k;
+            C.maxn = k < kClasses ? kClassN[k] : (uint32_t)cfg.max_n;
+            C.max_nnz = std::min<uint64_t>((uint64_t)cfg.max_graph_nnz, (uint64_t)C.maxn * (C.maxn - 1));
+            C.arena = std::min<uint64_t>((uint64_t)C.maxn * (C.maxn + 1) / 2, 64ull * C.maxn) + 64;
+            C.levels = std::min<uint32_t>(C.maxn + 2, 4096u);
+            const Layout L = make_layout(C.maxn, C.max_nnz, ring, C.arena, C.levels, k == kClasses);
+            C.scratch_stride = L.total;
+            int per_cu = 0;
+            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(k), kWave, 0));
+            if (per_cu <= 0) per_cu = 1;
+            // a share of the chip, so that launches of other slots stay resident
+            int64_t grid = (int64_t)per_cu * prop.multiProcessorCount / std::max(1, cfg.share);
+            if (k == kClasses) grid = std::min<int64_t>(grid, cfg.global_workers);
+            grid = std::max<int64_t>(grid, 1);
+            const uint64_t budget = cfg.scratch_bytes_per_class;
+            grid = std::min<int64_t>(grid, std::max<int64_t>(1, (int64_t)(budget / L.total)));
+            C.grid = grid;
+            GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)grid)));
+            if (smin > 0 && C.maxn >= smin) {
+                C.records = (uint32_t)std::min<uint64_t>((uint64_t)grid, (uint64_t)kMaxRecords);
+                const uint64_t per_item = make_spec_layout(C.maxn, 2).total - make_spec_layout(C.maxn, 1).total + 64;
+                C.spec_slots = (uint32_t)std::min<uint64_t>(
+                    C.maxn, std::max<uint64_t>(64, cfg.spec_bytes_per_class / C.records / per_item));
+                C.spec_stride = make_spec_layout(C.maxn, C.spec_slots).total;
+                GCL_HIP(hipMalloc(&C.spec.p, (size_t)(C.spec_stride * C.records)));
+                GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(C.spec_stride * C.records)));
+                const uint32_t nwr = (C.records + 63) / 64;
+                C.board_words = 2 * (size_t)nwr;
+                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned long long) * C.board_words));
+                std::vector<unsigned long long> masks(C.board_words, 0ull);
+                for (uint32_t r = 0; r < C.records; ++r) masks[r / 64] |= 1ull << (r % 64);
+                GCL_HIP(hipMemcpy(C.board.p, masks.data(), sizeof(unsigned long long) * masks.size(),
+                                  hipMemcpyHostToDevice));
+            }
+            GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+            GCL_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
+            classes.push_back(std::move(C));
+        }
+        return GCL_OK;
+    }
+
+    // class of a graph with n vertices among the provisioned ones (-1: none)
+    int class_of(int64_t n) const {
+        for (size_t i = 0; i < classes.size(); ++i)
+            if ((uint64_t)n <= classes[i].maxn) return (int)i;
+        return -1;
+    }
+
+    // Launches the batch already staged in h_vtx / h_row / h_col (G graphs;
+    // with colours in h_color when `colored`).
+    gcl_status launch(int32_t g_count, uint32_t max_aut, bool colored) {
+        GCL_HIP(hipSetDevice(0));
+        G = g_count;
+        total_n = G > 0 ? h_vtx[G] : 0;
+        total_nnz = G > 0 ? h_row[total_n] : 0;
+        if (G > cfg.max_graphs || total_n > cfg.max_vertices || total_nnz > cfg.max_nnz)
+            return fail(GCL_E_CAPACITY, "batch exceeds the slot capacity");
+        std::vector<std::vector<int32_t>> per(classes.size());
+        std::vector<unsigned int> big(classes.size(), 0);
+        const uint32_t smin = spec_min_n();
+        for (int32_t g = 0; g < G; ++g) {
+            const int64_t n = h_vtx[g + 1] - h_vtx[g];
+            const uint64_t nnz = (uint64_t)(h_row[h_vtx[g + 1]] - h_row[h_vtx[g]]);
+            const int k = class_of(n);
+            if (k < 0 || nnz > classes[(size_t)k].max_nnz)
+                return fail(GCL_E_CAPACITY, "graph %d (n=%lld) exceeds the slot capacity", g, (long long)n);
+            per[(size_t)k].push_back(g);
+            if (smin > 0 && (uint32_t)n >= smin && classes[(size_t)k].records) ++big[(size_t)k];
+        }
+        size_t off = 0;
+        for (size_t k = 0; k < classes.size(); ++k) {
+            auto& v = per[k];
+            std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
+                return h_vtx[x + 1] - h_vtx[x] > h_vtx[y + 1] - h_vtx[y];
+            });
+            std::copy(v.begin(), v.end(), h_items + off);
+            classes[k].count = (int32_t)v.size();
+            classes[k].item_off = off;
+            off += v.size();
+        }
+        if (colored) {
+            for (int32_t g = 0; g < G; ++g) {
+                const int64_t v0 = h_vtx[g], v1 = h_vtx[g + 1];
+                for (int64_t v = v0; v < v1; ++v) h_init[v] = (int32_t)(v - v0);
+                std::stable_sort(h_init + v0, h_init + v1,
+                                 [&](int32_t x, int32_t y) { return h_color[v0 + x] < h_color[v0 + y]; });
+            }
+            GCL_HIP(hipMemcpyAsync(color.p, h_color, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice, main));
+            GCL_HIP(hipMemcpyAsync(init.p, h_init, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice, main));
+        }
+        for (int i = 0; i < 64; ++i) h_ctr_init[i] = 0;
+        for (size_t k = 0; k < classes.size(); ++k) h_ctr_init[32 + classes[k].cls] = big[k];
+        GCL_HIP(hipMemcpyAsync(vtx.p, h_vtx, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, main));
+        GCL_HIP(hipMemcpyAsync(row.p, h_row, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice, main));
+        if (total_nnz > 0)
+            GCL_HIP(hipMemcpyAsync(col.p, h_col, sizeof(int32_t) * (size_t)total_nnz, hipMemcpyHostToDevice, main));
+        if (G > 0)
+            GCL_HIP(hipMemcpyAsync(items.p, h_items, sizeof(int32_t) * (size_t)G, hipMemcpyHostToDevice, main));
+        GCL_HIP(hipMemcpyAsync(ctr_init.p, h_ctr_init, sizeof(unsigned int) * 64, hipMemcpyHostToDevice, main));
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p,
+                           (const unsigned int*)ctr_init.p, 64);
+        GCL_HIP(hipGetLastError());
+        GCL_HIP(hipEventRecord(start, main));
+        for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
+            Cls& C = *it;
+            if (C.count == 0) continue;
+            KParams P;
+            memset(&P, 0, sizeof(P));
+            P.num_items = C.count;
+            P.items = (const int32_t*)items.p + C.item_off;
+            P.vtx_off = (const int64_t*)vtx.p;
+            P.row_ptr = (const int64_t*)row.p;
+            P.col = (const int32_t*)col.p;
+            P.init_order = colored ? (const int32_t*)init.p : nullptr;
+            P.color = colored ? (const int32_t*)color.p : nullptr;
+            P.out_label = (int32_t*)label.p;
+            P.out = (DeviceGraphResult*)out.p;
+            P.next_item = (unsigned int*)ctr.p + C.cls;
+            P.scratch = (uint8_t*)C.scratch.p;
+            P.scratch_stride = C.scratch_stride;
+            P.maxn = C.maxn;
+            P.max_nnz = C.max_nnz;
+            P.ring = ring;
+            P.max_aut = max_aut;
+            P.arena_cap = C.arena;
+            P.levels = C.levels;
+            P.step_budget = step_budget();
+            P.time_limit = time_limit_ticks();
+            const bool spec = C.records > 0;
+            P.spec = spec ? (uint8_t*)C.spec.p : nullptr;
+            P.spec_stride = C.spec_stride;
+            P.spec_slots = C.spec_slots;
+            P.spec_records = C.records;
+            P.spec_min_n = spec ? smin : 0u;
+            P.slot_free = spec ? (unsigned long long*)C.board.p : nullptr;
+            P.board_mask = spec ? (unsigned long long*)C.board.p + C.board_words / 2 : nullptr;
+            P.pending = (unsigned int*)ctr.p + 32 + C.cls;
+            // fewer workers than graphs plus root-child work is pointless
+            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(C.grid, spec ? C.grid : C.count));
+            GCL_HIP(hipStreamWaitEvent(C.stream, start, 0));
+            hipError_t e = launch_class(C.cls, P, (int)grid, C.stream);
+            if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+            GCL_HIP(hipEventRecord(C.done, C.stream));
+        }
+        for (auto& C : classes)
+            if (C.count) GCL_HIP(hipStreamWaitEvent(main, C.done, 0));
+        if (G > 0) {
+            GCL_HIP(hipMemcpyAsync(h_out, out.p, sizeof(DeviceGraphResult) * (size_t)G, hipMemcpyDeviceToHost, main));
+            if (total_n > 0)
+                GCL_HIP(hipMemcpyAsync(h_label, label.p, sizeof(int32_t) * (size_t)total_n, hipMemcpyDeviceToHost,
+                                       main));
+        }
+        GCL_HIP(hipEventRecord(done, main));
+        busy = true;
+        return GCL_OK;
+    }
+
+    // true once the launch and its result copies have completed
+    gcl_status poll(bool* finished) {
+        hipError_t e = hipEventQuery(done);
+        if (e == hipErrorNotReady) {
+            *finished = false;
+            return GCL_OK;
+        }
+        if (e != hipSuccess) return fail(GCL_E_DEVICE, "search launch failed: %s", hipGetErrorString(e));
+        *finished = true;
+        busy = false;
+        return GCL_OK;
+    }
+};
+
+gcl_status slot_create(const SlotConfig& c, DeviceSlot** out) {
+    DeviceSlot* s = new (std::nothrow) DeviceSlot();
+    if (!s) return fail(GCL_E_NOMEM, "out of host memory");
+    gcl_status st = s->create(c);
+    if (st != GCL_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return GCL_OK;
+}
+void slot_destroy(DeviceSlot* s) { delete s; }
+SlotStaging slot_staging(DeviceSlot* s) {
+    SlotStaging t;
+    t.vtx = s->h_vtx;
+    t.row = s->h_row;
+    t.col = s->h_col;
+    t.color = s->h_color;
+    t.label = s->h_label;
+    t.out = s->h_out;
+    return t;
+}
+int slot_class_of(const DeviceSlot* s, int64_t n, uint64_t nnz) {
+    const int k = s->class_of(n);
+    if (k < 0 || nnz > s->classes[(size_t)k].max_nnz) return -1;
+    return k;
+}
+gcl_status slot_launch(DeviceSlot* s, int32_t graphs, uint32_t max_aut, bool colored) {
+    return s->launch(graphs, max_aut, colored);
+}
+gcl_status slot_poll(DeviceSlot* s, bool* finished) { return s->poll(finished); }
+
 gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out) {
     DeviceBatch* db = new (std::nothrow) DeviceBatch();
     if (!db) return fail(GCL_E_NOMEM, "out of host memory");

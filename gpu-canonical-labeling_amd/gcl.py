@@ -12,6 +12,7 @@ Python fallback for any computation.
 """
 import ctypes as C
 import os
+import time
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -65,6 +66,22 @@ class _ClassStats(C.Structure):
                 ("csr_bytes", C.c_uint64)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_graphs", C.c_int32), ("max_vertices", C.c_int64),
+                ("max_nnz", C.c_int64), ("max_n", C.c_int32), ("max_graph_nnz", C.c_int64),
+                ("max_aut", C.c_uint32), ("queue_bytes", C.c_uint64)]
+
+
+class _ServerResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("verdict", C.c_int32), ("n", C.c_int32),
+                ("launch_graphs", C.c_int32), ("latency_ms", C.c_double), ("nodes", C.c_uint64)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "refused_busy", "launches", "graphs_launched",
+                                           "pending", "in_flight", "queued_bytes")]
+
+
 _lib = None
 
 # exported symbols and their signatures (restype, argtypes)
@@ -99,6 +116,13 @@ _SIGS = {
     "gcl_device_batch_class_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_graph_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_gen_mix_batch": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "gcl_server_config_default": (None, [C.c_void_p]),
+    "gcl_server_create": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gcl_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gcl_server_poll": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    "gcl_server_stats_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gcl_server_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "gcl_server_destroy": (None, [C.c_void_p]),
     "gcl_owned_batch_free": (None, [C.c_void_p]),
 }
 
@@ -467,3 +491,90 @@ def batch_canon(batch: Batch, max_aut=10000, engine="gpu", plain=False) -> Batch
 
 def device_count():
     return int(lib().gcl_device_count())
+
+
+def server_config(**kw):
+    """Default server configuration (gcl_server_config_default) with overrides."""
+    c = _ServerConfig()
+    lib().gcl_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Request server with continuous batching (gcl_server_*).
+
+    submit() queues a canonicalize request (graph) or an isomorphism test
+    (graph, other) and returns a ticket; poll() returns finished results as
+    dicts; labels of canonicalize requests are kept per ticket."""
+
+    def __init__(self, **config):
+        self._cfg = server_config(**config)
+        self.h = C.c_void_p()
+        _check(lib().gcl_server_create(C.byref(self._cfg), C.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, g: Graph, other: Graph = None, want_labels=True):
+        ca = g._csr()
+        cb = other._csr() if other is not None else None
+        lab = np.zeros(max(g.n, 1), np.int32) if want_labels else None
+        t = C.c_uint64()
+        _check(lib().gcl_server_submit(self.h, C.byref(ca), C.byref(cb) if cb is not None else None,
+                                       _ptr(lab) if lab is not None else None, C.byref(t)))
+        self._keep[t.value] = (g, other, lab)
+        return t.value
+
+    def poll(self, cap=1024, timeout_ms=0):
+        buf = (_ServerResult * cap)()
+        k = lib().gcl_server_poll(self.h, buf, cap, timeout_ms)
+        if k < 0:
+            _check(k)
+        out = []
+        for i in range(k):
+            r = {f: getattr(buf[i], f) for f, _ in _ServerResult._fields_}
+            g, other, lab = self._keep.pop(r["ticket"], (None, None, None))
+            if lab is not None and g is not None:
+                r["labels"] = lab[:g.n]
+            out.append(r)
+        return out
+
+    def wait_all(self, tickets, timeout_s=120.0):
+        """Results of the given tickets (dict ticket -> result), polling until all are done."""
+        want, got = set(tickets), {}
+        t_end = time.time() + timeout_s
+        while want - set(got):
+            if time.time() > t_end:
+                raise GclError(-7, "server results did not arrive in time")
+            for r in self.poll(timeout_ms=50):
+                got[r["ticket"]] = r
+        return got
+
+    def stats(self):
+        st = _ServerStats()
+        _check(lib().gcl_server_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def replay(self, batch: Batch, arrival_s, pair_with=None):
+        """Submits graph i at arrival_s[i] (paired with graph pair_with[i] when >= 0); returns per-request
+        arrays: status, verdict, latency_ms, nodes, launch_graphs."""
+        count = len(arrival_s)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        pw = None if pair_with is None else np.ascontiguousarray(pair_with, np.int32)
+        out = (_ServerResult * max(count, 1))()
+        c = batch._c()
+        _check(lib().gcl_server_replay(self.h, C.byref(c), _ptr(pw) if pw is not None else None, _ptr(arr), count,
+                                       out))
+        res = {f: np.array([getattr(out[i], f) for i in range(count)]) for f, _ in _ServerResult._fields_}
+        return res
+
+    def close(self):
+        if self.h:
+            lib().gcl_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

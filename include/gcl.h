@@ -252,6 +252,83 @@ gcl_status gcl_gen_mix_batch(int32_t count, uint64_t seed, int32_t n_min, int32_
                              int32_t threads, gcl_owned_batch** out);
 void gcl_owned_batch_free(gcl_owned_batch* b);
 
+/* ---------------------------------------------------------------- server */
+/*
+ * Request server: canonicalize and isomorphism requests submitted as they
+ * arrive, run with continuous batching on device 0.  Every device buffer
+ * (CSR arena, per-class worker scratch, speculation records) and the pinned
+ * host staging are allocated once at create: `slots` launch contexts, each
+ * with the per-launch capacity below.  A scheduler thread packs queued
+ * requests (in arrival order) into any free slot while the other slots' work
+ * runs on the device, and completes a slot's requests when its launch ends.
+ * Admission is by a footprint estimated from n and m: a request whose graph
+ * exceeds a slot's capacity is refused with GCL_E_CAPACITY, and one that
+ * would take the queued footprint past queue_bytes with GCL_E_BUSY (retry
+ * later).  No counterpart in the reference (its only entry point is the
+ * per-graph Nauty_Parallel call, src/nauty_parallel.h:9-13).
+ */
+typedef struct gcl_server_config {
+    int32_t slots;             /* launch contexts (>= 1); default 3 */
+    int32_t max_graphs;        /* graphs per launch; default 4096 */
+    int64_t max_vertices;      /* sum of n per launch; default 2^21 */
+    int64_t max_nnz;           /* sum of directed edges per launch; default 2^24 */
+    int32_t max_n;             /* largest graph accepted; default 4096 */
+    int64_t max_graph_nnz;     /* directed edges of the largest graph; default 2^20 */
+    uint32_t max_aut;          /* automorphism ring per search (>= 1); default 10000 */
+    uint64_t queue_bytes;      /* footprint budget of queued + running requests; default 2^30 */
+} gcl_server_config;
+
+/* Fills the defaults listed above. */
+void gcl_server_config_default(gcl_server_config* cfg);
+
+typedef struct gcl_server gcl_server;
+
+typedef struct gcl_server_result {
+    uint64_t ticket;
+    int32_t status;            /* gcl_status of the request */
+    int32_t verdict;           /* pair: 1 isomorphic, 0 not; canonicalize: -1 */
+    int32_t n;                 /* vertices of the first graph */
+    int32_t launch_graphs;     /* graphs in the launch that ran it */
+    double latency_ms;         /* submit (or scheduled arrival) to completion */
+    uint64_t nodes;            /* search-tree nodes (both graphs of a pair) */
+} gcl_server_result;
+
+typedef struct gcl_server_stats {
+    uint64_t submitted, completed, refused_busy, launches, graphs_launched;
+    uint64_t pending, in_flight, queued_bytes;
+} gcl_server_stats;
+
+gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out);
+
+/*
+ * Queues a request and returns its ticket.  b == NULL: canonicalize a;
+ * otherwise the isomorphism test of a and b.  The graphs are copied.
+ * out_label (n_a ints, or NULL) receives a's canonical labeling and must stay
+ * valid until the request's result has been polled.
+ */
+gcl_status gcl_server_submit(gcl_server* s, const gcl_csr* a, const gcl_csr* b,
+                             int32_t* out_label, uint64_t* ticket);
+
+/* Up to cap completed results, waiting up to timeout_ms (0: no wait) for
+ * the first.  Returns the number written, or a negative gcl_status. */
+int32_t gcl_server_poll(gcl_server* s, gcl_server_result* out, int32_t cap, int32_t timeout_ms);
+
+gcl_status gcl_server_stats_get(gcl_server* s, gcl_server_stats* st);
+
+/*
+ * Load replay: request i is graph i of `graphs`, paired with graph
+ * pair_with[i] when pair_with[i] >= 0 (NULL: all canonicalize), submitted at
+ * arrival_s[i] seconds after the call starts (non-decreasing); a request
+ * refused as busy is retried and its latency counts from the scheduled
+ * arrival.  Blocks until every request has completed; out[i] gets the result
+ * of request i.
+ */
+gcl_status gcl_server_replay(gcl_server* s, const gcl_batch* graphs, const int32_t* pair_with,
+                             const double* arrival_s, int32_t count, gcl_server_result* out);
+
+/* Completes the queued and running requests, then frees everything. */
+void gcl_server_destroy(gcl_server* s);
+
 /* ----------------------------------------------------------- device info */
 /* Number of visible GPUs (0 when none); never initialises a context when
  * there is no driver. */

@@ -1,0 +1,123 @@
+"""Request server (continuous batching) through the C ABI: results are the
+same alone and in the mix, isomorphism verdicts match the oracle, admission
+refuses what does not fit, and a Poisson replay completes every request."""
+import sys
+
+import numpy as np
+import pytest
+
+import gcl
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _mix(count, seed):
+    rs = np.random.RandomState(seed)
+    out = []
+    for i in range(count):
+        n = int(np.exp(rs.uniform(np.log(16), np.log(600))))
+        k = rs.randint(3)
+        if k == 0:
+            g = gcl.erdos_renyi_mean_degree(n, float(rs.uniform(3, 12)), seed * 1000 + i)
+        elif k == 1:
+            d = int(rs.choice([3, 4, 5]))
+            g = gcl.random_regular(n + (n * d) % 2, d, seed * 1000 + i)
+        else:
+            g = gcl.torus(int(rs.randint(4, 20)), int(rs.randint(4, 20)))
+        out.append(gcl.permute(g, seed * 7919 + i))
+    return out
+
+
+def test_server_same_alone_and_in_mix():
+    graphs = _mix(90, 5)
+    srv = gcl.Server(slots=3, max_n=1024, max_aut=64)
+    try:
+        _log("alone")
+        alone = {}
+        for i in (0, 7, 33, 60, 89):
+            t = srv.submit(graphs[i])
+            alone[i] = srv.wait_all([t])[t]["labels"].copy()
+        _log("mix")
+        tickets = [srv.submit(g) for g in graphs]
+        res = srv.wait_all(tickets)
+        for i, t in enumerate(tickets):
+            r = res[t]
+            assert r["status"] == 0, (i, r)
+            keys = gcl.canonical_keys(graphs[i], r["labels"])
+            o = oracle.canon_graph(graphs[i], max_aut=64)
+            assert np.array_equal(keys, o["keys"]), i
+            if i in alone:
+                assert np.array_equal(r["labels"], alone[i]), i
+        st = srv.stats()
+        assert st["completed"] >= 95 and st["pending"] == 0 and st["in_flight"] == 0
+        assert st["launches"] >= 2
+    finally:
+        srv.close()
+
+
+def test_server_pair_verdicts_match_oracle():
+    base = _mix(24, 9)
+    srv = gcl.Server(slots=2, max_n=1024)
+    try:
+        pairs = []
+        for i, g in enumerate(base):
+            other = gcl.permute(g, 500 + i) if i % 2 == 0 else gcl.edge_swap(g, 700 + i)
+            pairs.append((g, other))
+        tickets = [srv.submit(a, b) for a, b in pairs]
+        res = srv.wait_all(tickets)
+        for (a, b), t in zip(pairs, tickets):
+            want = np.array_equal(oracle.canon_graph(a)["keys"], oracle.canon_graph(b)["keys"]) if a.m == b.m else False
+            assert res[t]["status"] == 0
+            assert res[t]["verdict"] == (1 if want else 0), (a.n, res[t])
+    finally:
+        srv.close()
+
+
+def test_server_admission():
+    srv = gcl.Server(slots=1, max_n=256, max_graph_nnz=4096, queue_bytes=64 * 1024)
+    try:
+        big = gcl.random_regular(300, 3, 1)
+        with pytest.raises(gcl.GclError) as e:
+            srv.submit(big)
+        assert e.value.status == -6  # capacity
+        g = gcl.random_regular(200, 3, 2)
+        # fill the queue budget; every accepted request still completes
+        tickets, busy = [], 0
+        for i in range(64):
+            try:
+                tickets.append(srv.submit(gcl.permute(g, i)))
+            except gcl.GclError as e:
+                assert e.status == -8  # busy
+                busy += 1
+        assert busy > 0 and tickets
+        res = srv.wait_all(tickets)
+        assert all(r["status"] == 0 for r in res.values())
+        assert srv.stats()["refused_busy"] == busy
+    finally:
+        srv.close()
+
+
+def test_server_replay_poisson():
+    graphs = _mix(400, 13)
+    batch = gcl.Batch(graphs)
+    rs = np.random.RandomState(1)
+    arrival = np.cumsum(rs.exponential(1.0 / 4000.0, size=len(graphs)))
+    pair_with = np.full(len(graphs), -1, np.int32)
+    pair_with[::4] = np.arange(1, len(graphs), 4)[: len(pair_with[::4])]
+    srv = gcl.Server(slots=3, max_n=1024)
+    try:
+        _log("replay")
+        r = srv.replay(batch, arrival, pair_with)
+        assert np.all(r["status"] == 0)
+        assert np.all(r["latency_ms"] > 0)
+        assert np.all((r["verdict"] == -1) == (pair_with < 0))
+        st = srv.stats()
+        assert st["completed"] == len(graphs)
+    finally:
+        srv.close()
