@@ -94,6 +94,60 @@ def cpu_baseline(batch, budget_s, stride):
                        f"of single-thread algorithm time ({'reference sequential binary' if kind == 'reference' else 'oracle'})")
 
 
+def served(args):
+    """Served measurement: requests over the same mix of sizes, half
+    canonicalize and half isomorphism tests (partner: a permuted copy or a
+    one-swap variant, 50/50), through the request server.  Phase 1 submits
+    them all at once (capacity); phase 2 replays a Poisson stream at half that
+    rate (latency from arrival to completion)."""
+    R = args.serve_requests
+    t0 = time.time()
+    first = gcl.mix_batch(R, args.seed + 1)
+    rs = np.random.RandomState(args.seed + 2)
+    graphs = [first.graph(i) for i in range(R)]
+    pair_with = np.full(R, -1, np.int32)
+    partners = []
+    for i in range(R):
+        if rs.rand() < 0.5:
+            g = graphs[i]
+            partners.append(gcl.permute(g, 1000003 + i) if rs.rand() < 0.5 else gcl.edge_swap(g, 2000003 + i))
+            pair_with[i] = R + len(partners) - 1
+    batch = gcl.Batch(graphs + partners)
+    labelings = R + len(partners)
+    log(f"served workload: {R} requests ({len(partners)} pairs) built in {time.time() - t0:.1f}s")
+    srv = gcl.Server(slots=3, max_n=1024, max_aut=10000)
+    try:
+        srv.replay(batch, np.zeros(min(R, 512)), pair_with[:min(R, 512)])  # warm-up
+        t1 = time.perf_counter()
+        r1 = srv.replay(batch, np.zeros(R), pair_with)
+        cap_s = time.perf_counter() - t1
+        if np.any(r1["status"] != 0):
+            raise SystemExit("bench.py: served requests failed")
+        cap = labelings / cap_s
+        rate = 0.5 * R / cap_s
+        arrival = np.cumsum(np.random.RandomState(args.seed + 3).exponential(1.0 / rate, size=R))
+        t2 = time.perf_counter()
+        r2 = srv.replay(batch, arrival, pair_with)
+        lat_s = time.perf_counter() - t2
+        if np.any(r2["status"] != 0):
+            raise SystemExit("bench.py: served requests failed")
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = r2["latency_ms"]
+    log(f"served: capacity {cap:.0f} labelings/s ({cap_s:.2f}s), poisson {rate:.0f} req/s: "
+        f"p50 {np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms")
+    return {"value": round(cap, 2), "unit": "labelings/s",
+            "requests": R, "pair_requests": len(partners),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
+            "poisson_rate_req_s": round(rate, 1), "poisson_wall_s": round(lat_s, 3),
+            "launches": int(st["launches"]),
+            "workload": "same size mix (n log-uniform [16,1024]); half canonicalize, half pair tests "
+                        "(permuted copy or one-swap variant, 50/50); 3 launch slots; the 2% of n 10k-50k "
+                        "graphs of the serving configuration are not in this bounded run"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +160,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=6000,
+                    help="requests of the served measurement (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -189,9 +245,22 @@ def main():
             if not np.array_equal(a, b):
                 raise SystemExit(f"bench.py: plain and batched labelings differ on graph {i}")
         plain_rate = ps / t_plain
+        # per-graph latency of the plain path (one launch per graph)
+        plat = []
+        for j in range(min(ps, 64)):
+            g = sub.graph(j)
+            t2 = time.perf_counter()
+            gcl.canon_label(g, engine="gpu", max_aut=64)
+            plat.append((time.perf_counter() - t2) * 1e3)
         out["plain"] = {"value": round(plain_rate, 2), "unit": "labelings/s",
+                        "latency_ms": {"p50": round(float(np.percentile(plat, 50)), 3),
+                                       "p99": round(float(np.percentile(plat, 99)), 3)},
                         "sample": f"{ps} graphs evenly spaced over the batch, one launch each"}
         out["batched_vs_plain"] = round(value / plain_rate, 2)
+        if args.serve_requests > 0:
+            out["served"] = served(args)
+            out["served_vs_plain"] = round(out["served"]["value"] / plain_rate, 2)
+            out["p99_latency_ms"] = out["served"]["latency_ms"]["p99"]
         out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 512)
     print(json.dumps(out), flush=True)
 
