@@ -34,39 +34,59 @@ gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32
 void device_batch_destroy(DeviceBatch* db);
 int32_t device_batch_size(const DeviceBatch* db);
 
-// Launch slot of the server: fixed device buffers and pinned staging.
-struct SlotConfig {
-    int32_t max_graphs = 0;       // graphs per launch
-    int64_t max_vertices = 0;     // sum of n per launch
-    int64_t max_nnz = 0;          // sum of directed edges per launch
-    int32_t max_n = 0;            // largest graph
-    int64_t max_graph_nnz = 0;    // directed edges of the largest graph
-    uint32_t max_aut = 1;
-    uint32_t ring_cap = 64;       // automorphism slots per worker
-    int32_t share = 2;            // launches meant to be resident at once
-    int32_t global_workers = 256; // workers of the n > 4096 class
+// Persistent serving kernels (one per size class) and their request slots.
+struct ServeConfig {
+    int32_t max_n = 4096;             // largest graph
+    int64_t max_graph_nnz = 1 << 20;  // directed edges of the largest graph
+    uint32_t max_aut = 10000;
+    uint32_t ring_cap = 64;           // automorphism slots per worker
+    int32_t slots = 0;                // request slots per class (0: twice the workers)
+    uint64_t lifetime_ticks = ~0ull;  // device wall-clock ticks a worker serves at most
     uint64_t scratch_bytes_per_class = 2ull << 30;
-    uint64_t spec_bytes_per_class = 1ull << 30;
+    // speculation records: one per worker, each large enough for every root
+    // child of a class-sized graph (a record that holds fewer leaves the rest
+    // to the owning worker alone: n = 1024 needs ~4.3 GiB over 1,843 records)
+    uint64_t spec_bytes_per_class = 8ull << 30;
+    // a server is mostly idle between arrivals, so graphs from n = 32 on
+    // spread their root children over the idle workers (latency); a batch
+    // keeps every worker busy and publishes from n = 128 on
+    uint32_t spec_min_n = 32;
 };
-struct SlotStaging {  // pinned host arrays of a slot
-    int64_t* vtx;
-    int64_t* row;
+struct ServeClassInfo {
+    uint32_t maxn;
+    uint64_t max_nnz;
+    uint32_t slots;
+    int64_t workers;
+};
+struct ServeSlotView {  // host-coherent arrays of one request slot
+    int32_t* n;
+    int32_t* row;       // maxn + 1 local offsets
     int32_t* col;
     int32_t* color;
-    int32_t* label;
-    DeviceGraphResult* out;
+    int32_t* init;
+    int32_t* colored;
+    const int32_t* label;
+    const DeviceGraphResult* out;
+    uint32_t* done;     // set by the device once label and out are written
 };
-struct DeviceSlot;
-gcl_status slot_create(const SlotConfig& c, DeviceSlot** out);
-void slot_destroy(DeviceSlot* s);
-SlotStaging slot_staging(DeviceSlot* s);
-int slot_class_of(const DeviceSlot* s, int64_t n, uint64_t nnz);  // -1 when the graph does not fit
-gcl_status slot_launch(DeviceSlot* s, int32_t graphs, uint32_t max_aut, bool colored);  // staged batch
-gcl_status slot_poll(DeviceSlot* s, bool* finished);
+struct ServeEngine;
+gcl_status serve_create(const ServeConfig& c, ServeEngine** out);
+void serve_destroy(ServeEngine* s);  // stops the kernels (after the queues drain) and frees
+int serve_classes(const ServeEngine* s);
+ServeClassInfo serve_class_info(const ServeEngine* s, int k);
+int serve_class_of(const ServeEngine* s, int64_t n, int64_t nnz);  // -1: too large
+ServeSlotView serve_slot(ServeEngine* s, int k, uint32_t slot);
+void serve_publish(ServeEngine* s, int k, uint32_t slot);  // slot data written before
+void serve_stop(ServeEngine* s);
+gcl_status serve_wait(ServeEngine* s);
+gcl_status serve_check(ServeEngine* s);  // GCL_E_DEVICE once a serving kernel has failed
 
 int32_t device_count();
+uint64_t device_wall_ticks_per_ms();
 uint64_t step_budget();
-uint32_t spec_min_n();
+// Smallest graph whose root children are published to helpers
+// (GCL_SPEC_MIN_N overrides; 0 disables).
+uint32_t spec_min_n(uint32_t dflt = 128);
 uint64_t time_limit_ticks();
 
 }  // namespace gcl

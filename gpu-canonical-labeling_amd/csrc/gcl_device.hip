@@ -106,10 +106,17 @@ uint64_t time_limit_ticks() {
 
 // Graphs with at least this many vertices publish their root children for
 // idle workers (0 disables); GCL_SPEC_MIN_N overrides the default.
-uint32_t spec_min_n() {
+uint32_t spec_min_n(uint32_t dflt) {
     const char* e = getenv("GCL_SPEC_MIN_N");
     if (e && *e) return (uint32_t)strtoul(e, nullptr, 10);
-    return 128;
+    return dflt;
+}
+
+uint64_t device_wall_ticks_per_ms() {
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    return (uint64_t)rate_khz;
 }
 
 int32_t device_count() {
@@ -461,281 +468,272 @@ struct DeviceBatch {
     }
 };
 
-// ------------------------------------------------------------- launch slot
-// A launch context with fixed buffers, for the server: every device buffer,
-// the per-class worker scratch and speculation records, and pinned host
-// staging are allocated once; a batch within the capacity is staged, copied
-// and launched asynchronously with no allocation.
-struct DeviceSlot {
+// ----------------------------------------------------------------- serving
+// Persistent serving kernels: one LDS kernel for every graph up to
+// min(max_n, 1024) vertices and, when max_n is larger, a second one for the
+// rest.  A persistent kernel holds its hardware queue for its whole life and
+// kernels that share a queue run one after the other, so the server keeps to
+// two (HIP gives a process 4 queues by default, one of them taken by the
+// null stream, where the runtime's copy and fill kernels run).  For the same
+// reason every buffer is allocated and initialised before the first launch:
+// host-coherent queue and request slots (written by the host, read by the
+// kernels with system-scope loads; results written back the same way), and
+// the device arrays the claiming worker copies a request into.
+template <int MAXN>
+hipError_t launch_serve_lds(const KParams& P, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(serve_kernel_lds<MAXN>, dim3(grid), dim3(kWave), 0, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve(int cls, const KParams& P, int grid, hipStream_t s) {
+    switch (cls) {
+        case 0: return launch_serve_lds<64>(P, grid, s);
+        case 1: return launch_serve_lds<128>(P, grid, s);
+        case 2: return launch_serve_lds<256>(P, grid, s);
+        case 3: return launch_serve_lds<512>(P, grid, s);
+        case 4: return launch_serve_lds<1024>(P, grid, s);
+        case 5: return launch_serve_lds<2048>(P, grid, s);
+        case 6: return launch_serve_lds<4096>(P, grid, s);
+        default:
+            hipLaunchKernelGGL(serve_kernel_global, dim3(grid), dim3(kWave), 0, s, P);
+            return hipGetLastError();
+    }
+}
+
+const void* serve_kernel(int cls) {
+    switch (cls) {
+        case 0: return (const void*)serve_kernel_lds<64>;
+        case 1: return (const void*)serve_kernel_lds<128>;
+        case 2: return (const void*)serve_kernel_lds<256>;
+        case 3: return (const void*)serve_kernel_lds<512>;
+        case 4: return (const void*)serve_kernel_lds<1024>;
+        case 5: return (const void*)serve_kernel_lds<2048>;
+        case 6: return (const void*)serve_kernel_lds<4096>;
+        default: return (const void*)serve_kernel_global;
+    }
+}
+
+struct HostBuf {
+    void* p = nullptr;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    HostBuf(HostBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct ServeEngine {
     struct Cls {
-        int cls = -1;
-        uint32_t maxn = 0;
+        int cls = 0;
+        uint32_t maxn = 0, slots = 0, ring = 0;
         uint64_t max_nnz = 0;
         int64_t grid = 0;
-        uint32_t records = 0, spec_slots = 0;
-        uint64_t spec_stride = 0, scratch_stride = 0, arena = 0;
-        uint32_t levels = 0;
-        DevBuf scratch, spec, board;
-        size_t board_words = 0;
+        HostBuf q, ringb, hn, hrow, hcol, hcolor, hinit, hcolored, done, out, label;
+        DevBuf vtx, row, col, color, init, scratch, spec, board, ctr;
         hipStream_t stream = nullptr;
-        hipEvent_t done = nullptr;
-        int32_t count = 0;    // graphs of the current launch
-        size_t item_off = 0;
+        uint32_t tail = 0;  // host copy of the published count
     };
-    SlotConfig cfg;
-    uint32_t ring = 1;
-    DevBuf vtx, row, col, label, out, items, ctr, ctr_init, color, init;
-    // pinned staging (host side of the copies)
-    int64_t* h_vtx = nullptr;
-    int64_t* h_row = nullptr;
-    int32_t* h_col = nullptr;
-    int32_t* h_color = nullptr;  // per vertex; used when a launch has coloured graphs
-    int32_t* h_init = nullptr;   // per graph: vertices in (colour, id) order
-    int32_t* h_items = nullptr;
-    int32_t* h_label = nullptr;
-    DeviceGraphResult* h_out = nullptr;
-    unsigned int* h_ctr_init = nullptr;
     std::vector<Cls> classes;
-    std::vector<unsigned long long> board_init_h;
-    hipStream_t main = nullptr;
-    hipEvent_t start = nullptr, done = nullptr;
-    int32_t G = 0;
-    int64_t total_n = 0, total_nnz = 0;
-    bool busy = false;
+    bool running = false;
 
-    ~DeviceSlot() {
-        if (main) (void)hipStreamSynchronize(main);
-        for (auto& c : classes) {
-            if (c.done) (void)hipEventDestroy(c.done);
+    ~ServeEngine() {
+        stop();
+        (void)wait();
+        for (auto& c : classes)
             if (c.stream) (void)hipStreamDestroy(c.stream);
-        }
-        if (start) (void)hipEventDestroy(start);
-        if (done) (void)hipEventDestroy(done);
-        if (main) (void)hipStreamDestroy(main);
-        for (void* p : {(void*)h_vtx, (void*)h_row, (void*)h_col, (void*)h_items, (void*)h_label, (void*)h_out,
-                        (void*)h_ctr_init, (void*)h_color, (void*)h_init})
-            if (p) (void)hipHostFree(p);
     }
 
     template <class T>
-    gcl_status host_alloc(T** p, size_t count) {
-        GCL_HIP(hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault));
+    static gcl_status hmalloc(HostBuf& b, size_t count, T** ptr) {
+        GCL_HIP(hipHostMalloc(&b.p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocCoherent | hipHostMallocMapped));
+        memset(b.p, 0, sizeof(T) * std::max<size_t>(count, 1));
+        *ptr = (T*)b.p;
         return GCL_OK;
     }
 
-    gcl_status create(const SlotConfig& c) {
-        cfg = c;
-        if (cfg.max_graphs <= 0 || cfg.max_vertices <= 0 || cfg.max_nnz < 0 || cfg.max_n <= 0 ||
-            cfg.max_n > kMaxVertices || cfg.max_aut == 0)
-            return fail(GCL_E_INVALID, "invalid slot configuration");
+    gcl_status create(const ServeConfig& cfg) {
+        if (cfg.max_n <= 0 || cfg.max_n > kMaxVertices || cfg.max_graph_nnz < 0 || cfg.max_aut == 0)
+            return fail(GCL_E_INVALID, "invalid server configuration");
         if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
         GCL_HIP(hipSetDevice(0));
-        ring = std::max(1u, std::min(cfg.max_aut, cfg.ring_cap));
         hipDeviceProp_t prop;
         GCL_HIP(hipGetDeviceProperties(&prop, 0));
-        const size_t G1 = (size_t)cfg.max_graphs + 1, V1 = (size_t)cfg.max_vertices + 1;
-        GCL_HIP(hipMalloc(&vtx.p, sizeof(int64_t) * G1));
-        GCL_HIP(hipMalloc(&row.p, sizeof(int64_t) * V1));
-        GCL_HIP(hipMalloc(&col.p, sizeof(int32_t) * (size_t)std::max<int64_t>(cfg.max_nnz, 1)));
-        GCL_HIP(hipMalloc(&label.p, sizeof(int32_t) * V1));
-        GCL_HIP(hipMalloc(&out.p, sizeof(DeviceGraphResult) * G1));
-        GCL_HIP(hipMalloc(&items.p, sizeof(int32_t) * G1));
-        GCL_HIP(hipMalloc(&ctr.p, sizeof(unsigned int) * 64));
-        GCL_HIP(hipMalloc(&ctr_init.p, sizeof(unsigned int) * 64));
-        GCL_HIP(hipMalloc(&color.p, sizeof(int32_t) * V1));
-        GCL_HIP(hipMalloc(&init.p, sizeof(int32_t) * V1));
-        gcl_status st;
-        if ((st = host_alloc(&h_color, V1)) || (st = host_alloc(&h_init, V1))) return st;
-        if ((st = host_alloc(&h_vtx, G1)) || (st = host_alloc(&h_row, V1)) ||
-            (st = host_alloc(&h_col, (size_t)std::max<int64_t>(cfg.max_nnz, 1))) || (st = host_alloc(&h_items, G1)) ||
-            (st = host_alloc(&h_label, V1)) || (st = host_alloc(&h_out, G1)) || (st = host_alloc(&h_ctr_init, 64)))
-            return st;
-        GCL_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
-        GCL_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
-        GCL_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-        // classes provisioned up to the largest graph accepted
-        int last = kClasses;
-        for (int k = 0; k < kClasses; ++k)
-            if ((uint32_t)cfg.max_n <= kClassN[k]) {
-                last = k;
-                break;
-            }
-        const uint32_t smin = spec_min_n();
-        for (int k = 0; k <= last; ++k) {
+        const uint32_t ring_slots = std::max(1u, std::min(cfg.max_aut, cfg.ring_cap));
+        // serving kernels: the size class of min(max_n, 1024), and for larger
+        // graphs the class of max_n (2048, 4096 or the global one)
+        auto class_for = [](uint32_t n) {
+            for (int k = 0; k < kClasses; ++k)
+                if (n <= kClassN[k]) return k;
+            return (int)kClasses;
+        };
+        std::vector<int> ks = {class_for(std::min<uint32_t>((uint32_t)cfg.max_n, 1024u))};
+        if (cfg.max_n > 1024) ks.push_back(class_for((uint32_t)cfg.max_n));
+        const double share[2] = {ks.size() == 1 ? 0.9 : 0.7, 0.2};
+        const uint32_t smin = spec_min_n(cfg.spec_min_n);
+        std::vector<KParams> params;
+        for (size_t ki = 0; ki < ks.size(); ++ki) {
+            const int k = ks[ki];
             Cls C;
             C.cls = k;
             C.maxn = k < kClasses ? kClassN[k] : (uint32_t)cfg.max_n;
-            C.max_nnz = std::min<uint64_t>((uint64_t)cfg.max_graph_nnz, (uint64_t)C.maxn * (C.maxn - 1));
-            C.arena = std::min<uint64_t>((uint64_t)C.maxn * (C.maxn + 1) / 2, 64ull * C.maxn) + 64;
-            C.levels = std::min<uint32_t>(C.maxn + 2, 4096u);
-            const Layout L = make_layout(C.maxn, C.max_nnz, ring, C.arena, C.levels, k == kClasses);
-            C.scratch_stride = L.total;
+            // the first kernel's slots take up to 24 edges per vertex (twice the
+            // densest graphs of the benchmark mix); the second one max_graph_nnz
+            C.max_nnz = ki == 0 ? std::min<uint64_t>((uint64_t)std::max<int64_t>(cfg.max_graph_nnz, 1), 24ull * C.maxn)
+                                : (uint64_t)std::max<int64_t>(cfg.max_graph_nnz, 1);
+            const uint64_t arena = std::min<uint64_t>((uint64_t)C.maxn * (C.maxn + 1) / 2, 64ull * C.maxn) + 64;
+            const uint32_t levels = std::min<uint32_t>(C.maxn + 2, 4096u);
+            const Layout L = make_layout(C.maxn, C.max_nnz, ring_slots, arena, levels, k == kClasses);
             int per_cu = 0;
-            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(k), kWave, 0));
+            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel(k), kWave, 0));
             if (per_cu <= 0) per_cu = 1;
-            // a share of the chip, so that launches of other slots stay resident
-            int64_t grid = (int64_t)per_cu * prop.multiProcessorCount / std::max(1, cfg.share);
-            if (k == kClasses) grid = std::min<int64_t>(grid, cfg.global_workers);
-            grid = std::max<int64_t>(grid, 1);
-            const uint64_t budget = cfg.scratch_bytes_per_class;
-            grid = std::min<int64_t>(grid, std::max<int64_t>(1, (int64_t)(budget / L.total)));
+            int64_t grid = (int64_t)((double)per_cu * prop.multiProcessorCount * share[ki]);
+            grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)(cfg.scratch_bytes_per_class / L.total)));
             C.grid = grid;
+            C.slots = cfg.slots > 0 ? (uint32_t)cfg.slots
+                                    : (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(64, 2 * grid));
+            C.ring = 1;
+            while (C.ring < C.slots) C.ring <<= 1;
+            const size_t S = C.slots, n1 = (size_t)C.maxn + 1;
+            uint32_t* q;
+            uint32_t* ringp;
+            int32_t *hn, *hrow, *hcol, *hcolor, *hinit, *hcolored, *label;
+            uint32_t* done;
+            DeviceGraphResult* out;
+            gcl_status st;
+            if ((st = hmalloc(C.q, 64, &q)) || (st = hmalloc(C.ringb, C.ring, &ringp)) || (st = hmalloc(C.hn, S, &hn)) ||
+                (st = hmalloc(C.hrow, S * n1, &hrow)) || (st = hmalloc(C.hcol, S * C.max_nnz, &hcol)) ||
+                (st = hmalloc(C.hcolor, S * C.maxn, &hcolor)) || (st = hmalloc(C.hinit, S * C.maxn, &hinit)) ||
+                (st = hmalloc(C.hcolored, S, &hcolored)) || (st = hmalloc(C.done, S, &done)) ||
+                (st = hmalloc(C.out, 2 * S, &out)) || (st = hmalloc(C.label, S * n1, &label)))
+                return st;
+            GCL_HIP(hipMalloc(&C.vtx.p, sizeof(int64_t) * 2 * S));
+            GCL_HIP(hipMalloc(&C.row.p, sizeof(int64_t) * S * n1));
+            GCL_HIP(hipMalloc(&C.col.p, sizeof(int32_t) * S * C.max_nnz));
+            GCL_HIP(hipMalloc(&C.color.p, sizeof(int32_t) * S * C.maxn));
+            GCL_HIP(hipMalloc(&C.init.p, sizeof(int32_t) * S * C.maxn));
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)grid)));
-            if (smin > 0 && C.maxn >= smin) {
-                C.records = (uint32_t)std::min<uint64_t>((uint64_t)grid, (uint64_t)kMaxRecords);
-                const uint64_t per_item = make_spec_layout(C.maxn, 2).total - make_spec_layout(C.maxn, 1).total + 64;
-                C.spec_slots = (uint32_t)std::min<uint64_t>(
-                    C.maxn, std::max<uint64_t>(64, cfg.spec_bytes_per_class / C.records / per_item));
-                C.spec_stride = make_spec_layout(C.maxn, C.spec_slots).total;
-                GCL_HIP(hipMalloc(&C.spec.p, (size_t)(C.spec_stride * C.records)));
-                GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(C.spec_stride * C.records)));
-                const uint32_t nwr = (C.records + 63) / 64;
-                C.board_words = 2 * (size_t)nwr;
-                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned long long) * C.board_words));
-                std::vector<unsigned long long> masks(C.board_words, 0ull);
-                for (uint32_t r = 0; r < C.records; ++r) masks[r / 64] |= 1ull << (r % 64);
-                GCL_HIP(hipMemcpy(C.board.p, masks.data(), sizeof(unsigned long long) * masks.size(),
-                                  hipMemcpyHostToDevice));
-            }
-            GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
-            GCL_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
-            classes.push_back(std::move(C));
-        }
-        return GCL_OK;
-    }
-
-    // class of a graph with n vertices among the provisioned ones (-1: none)
-    int class_of(int64_t n) const {
-        for (size_t i = 0; i < classes.size(); ++i)
-            if ((uint64_t)n <= classes[i].maxn) return (int)i;
-        return -1;
-    }
-
-    // Launches the batch already staged in h_vtx / h_row / h_col (G graphs;
-    // with colours in h_color when `colored`).
-    gcl_status launch(int32_t g_count, uint32_t max_aut, bool colored) {
-        GCL_HIP(hipSetDevice(0));
-        G = g_count;
-        total_n = G > 0 ? h_vtx[G] : 0;
-        total_nnz = G > 0 ? h_row[total_n] : 0;
-        if (G > cfg.max_graphs || total_n > cfg.max_vertices || total_nnz > cfg.max_nnz)
-            return fail(GCL_E_CAPACITY, "batch exceeds the slot capacity");
-        std::vector<std::vector<int32_t>> per(classes.size());
-        std::vector<unsigned int> big(classes.size(), 0);
-        const uint32_t smin = spec_min_n();
-        for (int32_t g = 0; g < G; ++g) {
-            const int64_t n = h_vtx[g + 1] - h_vtx[g];
-            const uint64_t nnz = (uint64_t)(h_row[h_vtx[g + 1]] - h_row[h_vtx[g]]);
-            const int k = class_of(n);
-            if (k < 0 || nnz > classes[(size_t)k].max_nnz)
-                return fail(GCL_E_CAPACITY, "graph %d (n=%lld) exceeds the slot capacity", g, (long long)n);
-            per[(size_t)k].push_back(g);
-            if (smin > 0 && (uint32_t)n >= smin && classes[(size_t)k].records) ++big[(size_t)k];
-        }
-        size_t off = 0;
-        for (size_t k = 0; k < classes.size(); ++k) {
-            auto& v = per[k];
-            std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
-                return h_vtx[x + 1] - h_vtx[x] > h_vtx[y + 1] - h_vtx[y];
-            });
-            std::copy(v.begin(), v.end(), h_items + off);
-            classes[k].count = (int32_t)v.size();
-            classes[k].item_off = off;
-            off += v.size();
-        }
-        if (colored) {
-            for (int32_t g = 0; g < G; ++g) {
-                const int64_t v0 = h_vtx[g], v1 = h_vtx[g + 1];
-                for (int64_t v = v0; v < v1; ++v) h_init[v] = (int32_t)(v - v0);
-                std::stable_sort(h_init + v0, h_init + v1,
-                                 [&](int32_t x, int32_t y) { return h_color[v0 + x] < h_color[v0 + y]; });
-            }
-            GCL_HIP(hipMemcpyAsync(color.p, h_color, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice, main));
-            GCL_HIP(hipMemcpyAsync(init.p, h_init, sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice, main));
-        }
-        for (int i = 0; i < 64; ++i) h_ctr_init[i] = 0;
-        for (size_t k = 0; k < classes.size(); ++k) h_ctr_init[32 + classes[k].cls] = big[k];
-        GCL_HIP(hipMemcpyAsync(vtx.p, h_vtx, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, main));
-        GCL_HIP(hipMemcpyAsync(row.p, h_row, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice, main));
-        if (total_nnz > 0)
-            GCL_HIP(hipMemcpyAsync(col.p, h_col, sizeof(int32_t) * (size_t)total_nnz, hipMemcpyHostToDevice, main));
-        if (G > 0)
-            GCL_HIP(hipMemcpyAsync(items.p, h_items, sizeof(int32_t) * (size_t)G, hipMemcpyHostToDevice, main));
-        GCL_HIP(hipMemcpyAsync(ctr_init.p, h_ctr_init, sizeof(unsigned int) * 64, hipMemcpyHostToDevice, main));
-        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, main, (unsigned int*)ctr.p,
-                           (const unsigned int*)ctr_init.p, 64);
-        GCL_HIP(hipGetLastError());
-        GCL_HIP(hipEventRecord(start, main));
-        for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
-            Cls& C = *it;
-            if (C.count == 0) continue;
+            // work head, pending counter (never 0 while serving), the host
+            // queue's mirror (tail, stop) and the next host-poll tick
+            unsigned int ctr_h[8] = {0u, 0x40000000u, 0u, 0u, 0u, 0u, 0u, 0u};
+            GCL_HIP(hipMalloc(&C.ctr.p, sizeof(ctr_h)));
+            GCL_HIP(hipMemcpy(C.ctr.p, ctr_h, sizeof(ctr_h), hipMemcpyHostToDevice));
             KParams P;
             memset(&P, 0, sizeof(P));
-            P.num_items = C.count;
-            P.items = (const int32_t*)items.p + C.item_off;
-            P.vtx_off = (const int64_t*)vtx.p;
-            P.row_ptr = (const int64_t*)row.p;
-            P.col = (const int32_t*)col.p;
-            P.init_order = colored ? (const int32_t*)init.p : nullptr;
-            P.color = colored ? (const int32_t*)color.p : nullptr;
-            P.out_label = (int32_t*)label.p;
-            P.out = (DeviceGraphResult*)out.p;
-            P.next_item = (unsigned int*)ctr.p + C.cls;
+            P.vtx_off = (const int64_t*)C.vtx.p;
+            P.row_ptr = (const int64_t*)C.row.p;
+            P.col = (const int32_t*)C.col.p;
+            P.color = (const int32_t*)C.color.p;
+            P.init_order = (const int32_t*)C.init.p;
+            void* dout = nullptr;
+            void* dlabel = nullptr;
+            GCL_HIP(hipHostGetDevicePointer(&dout, C.out.p, 0));
+            GCL_HIP(hipHostGetDevicePointer(&dlabel, C.label.p, 0));
+            P.out = (DeviceGraphResult*)dout;
+            P.out_label = (int32_t*)dlabel;
+            P.next_item = (unsigned int*)C.ctr.p;
+            P.pending = (unsigned int*)C.ctr.p + 1;
             P.scratch = (uint8_t*)C.scratch.p;
-            P.scratch_stride = C.scratch_stride;
+            P.scratch_stride = L.total;
             P.maxn = C.maxn;
             P.max_nnz = C.max_nnz;
-            P.ring = ring;
-            P.max_aut = max_aut;
-            P.arena_cap = C.arena;
-            P.levels = C.levels;
+            P.ring = ring_slots;
+            P.max_aut = cfg.max_aut;
+            P.arena_cap = arena;
+            P.levels = levels;
             P.step_budget = step_budget();
-            P.time_limit = time_limit_ticks();
-            const bool spec = C.records > 0;
-            P.spec = spec ? (uint8_t*)C.spec.p : nullptr;
-            P.spec_stride = C.spec_stride;
-            P.spec_slots = C.spec_slots;
-            P.spec_records = C.records;
-            P.spec_min_n = spec ? smin : 0u;
-            P.slot_free = spec ? (unsigned long long*)C.board.p : nullptr;
-            P.board_mask = spec ? (unsigned long long*)C.board.p + C.board_words / 2 : nullptr;
-            P.pending = (unsigned int*)ctr.p + 32 + C.cls;
-            // fewer workers than graphs plus root-child work is pointless
-            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(C.grid, spec ? C.grid : C.count));
-            GCL_HIP(hipStreamWaitEvent(C.stream, start, 0));
-            hipError_t e = launch_class(C.cls, P, (int)grid, C.stream);
-            if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
-            GCL_HIP(hipEventRecord(C.done, C.stream));
+            P.time_limit = cfg.lifetime_ticks;
+            if (smin > 0 && C.maxn >= smin) {
+                const uint32_t records = (uint32_t)std::min<int64_t>(grid, (int64_t)kMaxRecords);
+                const uint64_t per_item = make_spec_layout(C.maxn, 2).total - make_spec_layout(C.maxn, 1).total + 64;
+                const uint32_t spec_slots = (uint32_t)std::min<uint64_t>(
+                    C.maxn, std::max<uint64_t>(64, cfg.spec_bytes_per_class / records / per_item));
+                const uint64_t stride = make_spec_layout(C.maxn, spec_slots).total;
+                GCL_HIP(hipMalloc(&C.spec.p, (size_t)(stride * records)));
+                GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(stride * records)));
+                const uint32_t nwr = (records + 63) / 64;
+                std::vector<unsigned long long> masks(2 * (size_t)nwr, 0ull);
+                for (uint32_t r = 0; r < records; ++r) masks[r / 64] |= 1ull << (r % 64);
+                GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned long long) * masks.size()));
+                GCL_HIP(hipMemcpy(C.board.p, masks.data(), sizeof(unsigned long long) * masks.size(),
+                                  hipMemcpyHostToDevice));
+                P.spec = (uint8_t*)C.spec.p;
+                P.spec_stride = stride;
+                P.spec_slots = spec_slots;
+                P.spec_records = records;
+                P.spec_min_n = smin;
+                P.slot_free = (unsigned long long*)C.board.p;
+                P.board_mask = (unsigned long long*)C.board.p + nwr;
+            }
+            void* dp = nullptr;
+            ServeParams& V = P.sv;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.q.p, 0));
+            V.tail = (const uint32_t*)dp;
+            V.stop = (const uint32_t*)dp + 16;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.ringb.p, 0));
+            V.ring = (const uint32_t*)dp;
+            V.ring_mask = C.ring - 1;
+            V.slots = C.slots;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hn.p, 0));
+            V.h_n = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hrow.p, 0));
+            V.h_row = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hcol.p, 0));
+            V.h_col = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hcolor.p, 0));
+            V.h_color = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hinit.p, 0));
+            V.h_init = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.hcolored.p, 0));
+            V.h_colored = (const int32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.done.p, 0));
+            V.done = (uint32_t*)dp;
+            V.tail_seen = (uint32_t*)C.ctr.p + 2;
+            V.stop_seen = (uint32_t*)C.ctr.p + 3;
+            V.poll_at = (unsigned long long*)((uint32_t*)C.ctr.p + 4);
+            // one host read per 20 us at most
+            V.poll_ticks = (uint32_t)std::max<uint64_t>(1, device_wall_ticks_per_ms() / 50);
+            GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+            params.push_back(P);
+            classes.push_back(std::move(C));
         }
-        for (auto& C : classes)
-            if (C.count) GCL_HIP(hipStreamWaitEvent(main, C.done, 0));
-        if (G > 0) {
-            GCL_HIP(hipMemcpyAsync(h_out, out.p, sizeof(DeviceGraphResult) * (size_t)G, hipMemcpyDeviceToHost, main));
-            if (total_n > 0)
-                GCL_HIP(hipMemcpyAsync(h_label, label.p, sizeof(int32_t) * (size_t)total_n, hipMemcpyDeviceToHost,
-                                       main));
+        // everything is in place before the first persistent launch
+        GCL_HIP(hipDeviceSynchronize());
+        for (size_t ki = 0; ki < classes.size(); ++ki) {
+            hipError_t e = launch_serve(classes[ki].cls, params[ki], (int)classes[ki].grid, classes[ki].stream);
+            if (e != hipSuccess) return fail(GCL_E_DEVICE, "serving kernel launch failed: %s", hipGetErrorString(e));
+            running = true;
         }
-        GCL_HIP(hipEventRecord(done, main));
-        busy = true;
         return GCL_OK;
     }
 
-    // true once the launch and its result copies have completed
-    gcl_status poll(bool* finished) {
-        hipError_t e = hipEventQuery(done);
-        if (e == hipErrorNotReady) {
-            *finished = false;
-            return GCL_OK;
+    void stop() {
+        for (auto& c : classes)
+            if (c.q.p) __atomic_store_n((uint32_t*)c.q.p + 16, 1u, __ATOMIC_RELEASE);
+    }
+
+    gcl_status wait() {
+        gcl_status r = GCL_OK;
+        for (auto& c : classes)
+            if (c.stream && hipStreamSynchronize(c.stream) != hipSuccess && r == GCL_OK)
+                r = fail(GCL_E_DEVICE, "serving kernel failed");
+        running = false;
+        return r;
+    }
+
+    gcl_status check() {
+        for (auto& c : classes) {
+            const hipError_t e = hipStreamQuery(c.stream);
+            if (e != hipSuccess && e != hipErrorNotReady)
+                return fail(GCL_E_DEVICE, "serving kernel failed: %s", hipGetErrorString(e));
         }
-        if (e != hipSuccess) return fail(GCL_E_DEVICE, "search launch failed: %s", hipGetErrorString(e));
-        *finished = true;
-        busy = false;
         return GCL_OK;
     }
 };
 
-gcl_status slot_create(const SlotConfig& c, DeviceSlot** out) {
-    DeviceSlot* s = new (std::nothrow) DeviceSlot();
+gcl_status serve_create(const ServeConfig& c, ServeEngine** out) {
+    ServeEngine* s = new (std::nothrow) ServeEngine();
     if (!s) return fail(GCL_E_NOMEM, "out of host memory");
     gcl_status st = s->create(c);
     if (st != GCL_OK) {
@@ -745,26 +743,48 @@ gcl_status slot_create(const SlotConfig& c, DeviceSlot** out) {
     *out = s;
     return GCL_OK;
 }
-void slot_destroy(DeviceSlot* s) { delete s; }
-SlotStaging slot_staging(DeviceSlot* s) {
-    SlotStaging t;
-    t.vtx = s->h_vtx;
-    t.row = s->h_row;
-    t.col = s->h_col;
-    t.color = s->h_color;
-    t.label = s->h_label;
-    t.out = s->h_out;
-    return t;
+void serve_destroy(ServeEngine* s) { delete s; }
+int serve_classes(const ServeEngine* s) { return (int)s->classes.size(); }
+ServeClassInfo serve_class_info(const ServeEngine* s, int k) {
+    const auto& c = s->classes[(size_t)k];
+    ServeClassInfo i;
+    i.maxn = c.maxn;
+    i.max_nnz = c.max_nnz;
+    i.slots = c.slots;
+    i.workers = c.grid;
+    return i;
 }
-int slot_class_of(const DeviceSlot* s, int64_t n, uint64_t nnz) {
-    const int k = s->class_of(n);
-    if (k < 0 || nnz > s->classes[(size_t)k].max_nnz) return -1;
-    return k;
+int serve_class_of(const ServeEngine* s, int64_t n, int64_t nnz) {
+    for (size_t k = 0; k < s->classes.size(); ++k)
+        if ((uint64_t)n <= s->classes[k].maxn && (uint64_t)nnz <= s->classes[k].max_nnz) return (int)k;
+    return -1;
 }
-gcl_status slot_launch(DeviceSlot* s, int32_t graphs, uint32_t max_aut, bool colored) {
-    return s->launch(graphs, max_aut, colored);
+ServeSlotView serve_slot(ServeEngine* s, int k, uint32_t slot) {
+    auto& c = s->classes[(size_t)k];
+    const size_t n1 = (size_t)c.maxn + 1;
+    ServeSlotView v;
+    v.n = (int32_t*)c.hn.p + slot;
+    v.row = (int32_t*)c.hrow.p + slot * n1;
+    v.col = (int32_t*)c.hcol.p + slot * c.max_nnz;
+    v.color = (int32_t*)c.hcolor.p + slot * (size_t)c.maxn;
+    v.init = (int32_t*)c.hinit.p + slot * (size_t)c.maxn;
+    v.colored = (int32_t*)c.hcolored.p + slot;
+    v.label = (const int32_t*)c.label.p + slot * n1;
+    v.out = (const DeviceGraphResult*)c.out.p + 2 * (size_t)slot;
+    v.done = (uint32_t*)c.done.p + slot;
+    return v;
 }
-gcl_status slot_poll(DeviceSlot* s, bool* finished) { return s->poll(finished); }
+void serve_publish(ServeEngine* s, int k, uint32_t slot) {
+    auto& c = s->classes[(size_t)k];
+    uint32_t* ring = (uint32_t*)c.ringb.p;
+    __atomic_store_n(&ring[c.tail & (c.ring - 1)], slot, __ATOMIC_RELAXED);
+    ++c.tail;
+    // the slot's data and ring entry before the count that makes them visible
+    __atomic_store_n((uint32_t*)c.q.p, c.tail, __ATOMIC_RELEASE);
+}
+void serve_stop(ServeEngine* s) { s->stop(); }
+gcl_status serve_wait(ServeEngine* s) { return s->wait(); }
+gcl_status serve_check(ServeEngine* s) { return s->check(); }
 
 gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out) {
     DeviceBatch* db = new (std::nothrow) DeviceBatch();

@@ -49,9 +49,27 @@ Graph copy_graph(const gcl_csr& c) {
 
 struct Server {
     gcl_server_config cfg;
-    std::vector<DeviceSlot*> slots;
-    std::vector<std::vector<Request*>> running;  // requests of each busy slot
-    std::vector<int32_t> running_graphs;
+    ServeEngine* eng = nullptr;
+    struct Slot {
+        Request* req = nullptr;
+        int part = 0;  // graph 0 or 1 of the request
+    };
+    struct ClassState {
+        std::vector<uint32_t> free_slots;
+        std::vector<Slot> slots;
+        std::vector<uint32_t> busy;  // slots in flight
+    };
+    std::vector<ClassState> cls;
+    struct Live {
+        int remaining = 0;
+        int32_t status = 0;
+        uint64_t nodes = 0, dev_ticks = 0;
+        const int32_t* label[2] = {nullptr, nullptr};
+        std::vector<int32_t> lab[2];
+        Clock::time_point t_pub;
+    };
+    std::unordered_map<Request*, Live> live;
+    uint64_t ticks_per_ms = 100000;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
@@ -70,35 +88,32 @@ struct Server {
         }
         cv_work.notify_all();
         if (thr.joinable()) thr.join();
-        for (auto* s : slots) slot_destroy(s);
+        if (eng) serve_destroy(eng);
         for (auto* r : pending) delete r;
-        for (auto& v : running)
-            for (auto* r : v) delete r;
+        for (auto& kv : live) delete kv.first;
     }
 
     gcl_status create(const gcl_server_config& c) {
         cfg = c;
-        if (cfg.slots < 1 || cfg.max_graphs < 2 || cfg.max_vertices < 1 || cfg.max_nnz < 0 || cfg.max_n < 1 ||
-            cfg.max_n > kMaxVertices || cfg.max_graph_nnz < 0 || cfg.max_aut == 0 || cfg.queue_bytes == 0)
+        if (cfg.max_n < 1 || cfg.max_n > kMaxVertices || cfg.max_graph_nnz < 0 || cfg.max_aut == 0 ||
+            cfg.queue_bytes == 0 || cfg.slots < 0 || cfg.lifetime_ms < 0)
             return fail(GCL_E_INVALID, "invalid server configuration");
-        if ((int64_t)cfg.max_n * 2 > cfg.max_vertices)
-            return fail(GCL_E_INVALID, "max_vertices must hold a pair of the largest graphs");
-        SlotConfig sc;
-        sc.max_graphs = cfg.max_graphs;
-        sc.max_vertices = cfg.max_vertices;
-        sc.max_nnz = cfg.max_nnz;
+        ServeConfig sc;
         sc.max_n = cfg.max_n;
         sc.max_graph_nnz = cfg.max_graph_nnz;
         sc.max_aut = cfg.max_aut;
-        sc.share = std::min(cfg.slots, 2);
-        for (int i = 0; i < cfg.slots; ++i) {
-            DeviceSlot* s = nullptr;
-            gcl_status r = slot_create(sc, &s);
-            if (r != GCL_OK) return r;
-            slots.push_back(s);
+        sc.slots = cfg.slots;
+        if (cfg.lifetime_ms > 0) sc.lifetime_ticks = device_wall_ticks_per_ms() * (uint64_t)cfg.lifetime_ms;
+        ticks_per_ms = device_wall_ticks_per_ms();
+        gcl_status r = serve_create(sc, &eng);
+        if (r != GCL_OK) return r;
+        cls.resize((size_t)serve_classes(eng));
+        for (size_t k = 0; k < cls.size(); ++k) {
+            const ServeClassInfo ci = serve_class_info(eng, (int)k);
+            cls[k].slots.resize(ci.slots);
+            for (uint32_t s = ci.slots; s-- > 0;) cls[k].free_slots.push_back(s);
         }
-        running.resize(slots.size());
-        running_graphs.assign(slots.size(), 0);
+        st.launches = cls.size();
         thr = std::thread([this] { loop(); });
         return GCL_OK;
     }
@@ -111,17 +126,12 @@ struct Server {
         const gcl_csr* gs[2] = {a, b};
         const int ng = b ? 2 : 1;
         uint64_t fp = 0;
-        int64_t nsum = 0, msum = 0;
         for (int i = 0; i < ng; ++i) {
-            if (gs[i]->n > cfg.max_n || gs[i]->nnz > cfg.max_graph_nnz)
+            if (serve_class_of(eng, gs[i]->n, gs[i]->nnz) < 0)
                 return fail(GCL_E_CAPACITY, "graph (n=%d, nnz=%lld) exceeds the server's max_n=%d / max_graph_nnz=%lld",
                             gs[i]->n, (long long)gs[i]->nnz, cfg.max_n, (long long)cfg.max_graph_nnz);
             fp += graph_footprint(gs[i]->n, gs[i]->nnz);
-            nsum += gs[i]->n;
-            msum += gs[i]->nnz;
         }
-        if (nsum > cfg.max_vertices || msum > cfg.max_nnz)
-            return fail(GCL_E_CAPACITY, "request exceeds the per-launch capacity");
         if (fp > cfg.queue_bytes) return fail(GCL_E_CAPACITY, "request footprint exceeds queue_bytes");
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -150,86 +160,126 @@ struct Server {
         return GCL_OK;
     }
 
-    // Packs pending requests (arrival order) into slot i; returns the graphs staged.
-    int32_t stage(size_t i, std::vector<Request*>& taken, bool& colored) {
-        SlotStaging t = slot_staging(slots[i]);
-        int32_t G = 0;
-        int64_t V = 0, E = 0;
-        t.vtx[0] = 0;
-        t.row[0] = 0;
-        colored = false;
-        std::lock_guard<std::mutex> lk(mu);
-        while (!pending.empty()) {
-            Request* q = pending.front();
-            int64_t qn = 0, qm = 0;
-            for (int k = 0; k < q->ngraphs; ++k) {
-                qn += q->g[k].n;
-                qm += q->g[k].row_ptr[q->g[k].n];
-            }
-            if (G + q->ngraphs > cfg.max_graphs || V + qn > cfg.max_vertices || E + qm > cfg.max_nnz) break;
-            pending.pop_front();
-            for (int k = 0; k < q->ngraphs; ++k) {
-                const Graph& g = q->g[k];
-                for (int32_t v = 0; v < g.n; ++v) t.row[V + v + 1] = E + g.row_ptr[v + 1];
-                std::copy(g.col.begin(), g.col.end(), t.col + E);
-                if (!g.color.empty()) colored = true;
-                for (int32_t v = 0; v < g.n; ++v) t.color[V + v] = g.color.empty() ? 0 : g.color[v];
-                V += g.n;
-                E += g.row_ptr[g.n];
-                t.vtx[++G] = V;
-            }
-            taken.push_back(q);
+    // Writes graph `part` of q into slot s of class k (host-coherent memory).
+    void fill(int k, uint32_t s, Request* q, int part) {
+        const Graph& g = q->g[part];
+        ServeSlotView v = serve_slot(eng, k, s);
+        std::copy(g.row_ptr.begin(), g.row_ptr.end(), v.row);
+        std::copy(g.col.begin(), g.col.end(), v.col);
+        if (!g.color.empty()) {
+            std::copy(g.color.begin(), g.color.end(), v.color);
+            for (int32_t i = 0; i < g.n; ++i) v.init[i] = i;
+            std::stable_sort(v.init, v.init + g.n, [&](int32_t x, int32_t y) { return g.color[x] < g.color[y]; });
+            *v.colored = 1;
+        } else {
+            *v.colored = 0;
         }
-        return G;
+        __atomic_store_n(v.done, 0u, __ATOMIC_RELAXED);
+        __atomic_store_n(v.n, g.n, __ATOMIC_RELAXED);
+        cls[(size_t)k].slots[s] = Slot{q, part};
+        cls[(size_t)k].busy.push_back(s);
+        serve_publish(eng, k, s);
     }
 
-    // Results of slot i, whose launch has finished.
-    void complete(size_t i) {
-        SlotStaging t = slot_staging(slots[i]);
-        std::vector<gcl_server_result> out;
-        int32_t g = 0;
-        const auto now = Clock::now();
-        for (Request* q : running[i]) {
-            gcl_server_result r{};
-            r.ticket = q->ticket;
-            r.n = q->g[0].n;
-            r.launch_graphs = running_graphs[i];
-            r.verdict = -1;
-            int32_t status = 0;
-            const int32_t* lab[2] = {nullptr, nullptr};
-            for (int k = 0; k < q->ngraphs; ++k, ++g) {
-                const DeviceGraphResult& d = t.out[g];
-                if (d.status != 0 && status == 0) status = d.status;
-                r.nodes += d.nodes;
-                lab[k] = t.label + t.vtx[g];
+    // Publishes pending requests (arrival order) while their classes have free slots.
+    void dispatch() {
+        for (;;) {
+            Request* q;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (pending.empty()) return;
+                q = pending.front();
             }
-            if (status == 0 && q->out_label) std::copy(lab[0], lab[0] + q->g[0].n, q->out_label);
-            if (status == 0 && q->ngraphs == 2) {
-                const gcl_csr A = q->g[0].view(), B = q->g[1].view();
-                if (A.n != B.n || A.nnz != B.nnz) {
-                    r.verdict = 0;
-                } else {
-                    std::vector<uint32_t> ka((size_t)A.nnz), kb((size_t)B.nnz);
-                    canonical_keys(A, lab[0], ka.data());
-                    canonical_keys(B, lab[1], kb.data());
-                    r.verdict = ka == kb ? 1 : 0;
-                }
+            int k[2] = {-1, -1};
+            for (int i = 0; i < q->ngraphs; ++i) k[i] = serve_class_of(eng, q->g[i].n, q->g[i].row_ptr[q->g[i].n]);
+            size_t need0 = 1, need1 = q->ngraphs == 2 ? 1 : 0;
+            if (q->ngraphs == 2 && k[0] == k[1]) {
+                need0 = 2;
+                need1 = 0;
             }
-            r.status = status;
-            r.latency_ms = std::chrono::duration<double, std::milli>(now - q->t0).count();
-            out.push_back(r);
-        }
-        {
+            if (cls[(size_t)k[0]].free_slots.size() < need0) return;
+            if (need1 && cls[(size_t)k[1]].free_slots.size() < need1) return;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                pending.pop_front();
+            }
+            Live& L = live[q];
+            L.remaining = q->ngraphs;
+            L.t_pub = Clock::now();
+            for (int i = 0; i < q->ngraphs; ++i) {
+                auto& fs = cls[(size_t)k[i]].free_slots;
+                const uint32_t s = fs.back();
+                fs.pop_back();
+                fill(k[i], s, q, i);
+            }
             std::lock_guard<std::mutex> lk(mu);
-            for (Request* q : running[i]) {
-                queued_bytes -= q->footprint;
-                delete q;
-            }
-            running[i].clear();
-            for (auto& r : out) done.push_back(r);
-            st.completed += out.size();
+            st.graphs_launched += (uint64_t)q->ngraphs;
         }
-        cv_done.notify_all();
+    }
+
+    void finish(Request* q, Live& L) {
+        gcl_server_result r{};
+        r.ticket = q->ticket;
+        r.n = q->g[0].n;
+        r.verdict = -1;
+        r.status = L.status;
+        r.nodes = L.nodes;
+        r.device_ms = (double)L.dev_ticks / (double)ticks_per_ms;
+        if (L.status == 0 && q->out_label) std::copy(L.lab[0].begin(), L.lab[0].end(), q->out_label);
+        if (L.status == 0 && q->ngraphs == 2) {
+            const gcl_csr A = q->g[0].view(), B = q->g[1].view();
+            if (A.n != B.n || A.nnz != B.nnz) {
+                r.verdict = 0;
+            } else {
+                std::vector<uint32_t> ka((size_t)A.nnz), kb((size_t)B.nnz);
+                canonical_keys(A, L.lab[0].data(), ka.data());
+                canonical_keys(B, L.lab[1].data(), kb.data());
+                r.verdict = ka == kb ? 1 : 0;
+            }
+        }
+        const auto now = Clock::now();
+        r.latency_ms = std::chrono::duration<double, std::milli>(now - q->t0).count();
+        r.queue_ms = std::chrono::duration<double, std::milli>(L.t_pub - q->t0).count();
+        std::lock_guard<std::mutex> lk(mu);
+        queued_bytes -= q->footprint;
+        done.push_back(r);
+        ++st.completed;
+    }
+
+    // Collects slots whose done flag the device has set; returns how many.
+    size_t collect() {
+        size_t got = 0;
+        std::vector<Request*> finished;
+        for (size_t k = 0; k < cls.size(); ++k) {
+            auto& C = cls[k];
+            for (size_t i = 0; i < C.busy.size();) {
+                const uint32_t s = C.busy[i];
+                ServeSlotView v = serve_slot(eng, (int)k, s);
+                if (__atomic_load_n(v.done, __ATOMIC_ACQUIRE) == 0u) {
+                    ++i;
+                    continue;
+                }
+                Slot sl = C.slots[s];
+                Live& L = live[sl.req];
+                const DeviceGraphResult& d = *v.out;
+                if (d.status != 0 && L.status == 0) L.status = d.status;
+                L.nodes += d.nodes;
+                L.dev_ticks += d.device_ns;
+                L.lab[sl.part].assign(v.label, v.label + sl.req->g[sl.part].n);
+                if (--L.remaining == 0) finished.push_back(sl.req);
+                C.busy[i] = C.busy.back();
+                C.busy.pop_back();
+                C.free_slots.push_back(s);
+                ++got;
+            }
+        }
+        for (Request* q : finished) {
+            finish(q, live[q]);
+            live.erase(q);
+            delete q;
+        }
+        if (!finished.empty()) cv_done.notify_all();
+        return got;
     }
 
     void fail_all(gcl_status e) {
@@ -246,10 +296,9 @@ struct Server {
             ++st.completed;
             delete q;
         };
-        for (auto& v : running) {
-            for (Request* q : v) drop(q);
-            v.clear();
-        }
+        for (auto& kv : live) drop(kv.first);
+        live.clear();
+        for (auto& C : cls) C.busy.clear();
         while (!pending.empty()) {
             drop(pending.front());
             pending.pop_front();
@@ -258,48 +307,33 @@ struct Server {
     }
 
     void loop() {
-        std::vector<bool> busy(slots.size(), false);
+        auto last_check = Clock::now();
+        Clock::time_point stop_at{};
         for (;;) {
-            bool any_busy = false;
-            for (size_t i = 0; i < slots.size(); ++i) {
-                if (!busy[i]) continue;
-                bool fin = false;
-                if (slot_poll(slots[i], &fin) != GCL_OK) {
+            const size_t got = collect();
+            dispatch();
+            bool inflight = false;
+            for (auto& C : cls) inflight = inflight || !C.busy.empty();
+            if (inflight && Clock::now() - last_check > std::chrono::milliseconds(20)) {
+                last_check = Clock::now();
+                if (serve_check(eng) != GCL_OK) {
                     fail_all(GCL_E_DEVICE);
                     return;
                 }
-                if (fin) {
-                    complete(i);
-                    busy[i] = false;
-                } else {
-                    any_busy = true;
-                }
-            }
-            for (size_t i = 0; i < slots.size(); ++i) {
-                if (busy[i]) continue;
-                std::vector<Request*> taken;
-                bool colored = false;
-                const int32_t G = stage(i, taken, colored);
-                if (G == 0) break;
-                running[i] = std::move(taken);
-                running_graphs[i] = G;
-                if (slot_launch(slots[i], G, cfg.max_aut, colored) != GCL_OK) {
-                    fail_all(GCL_E_DEVICE);
-                    return;
-                }
-                busy[i] = any_busy = true;
-                std::lock_guard<std::mutex> lk(mu);
-                ++st.launches;
-                st.graphs_launched += (uint64_t)G;
             }
             std::unique_lock<std::mutex> lk(mu);
-            if (!any_busy) {
-                if (stop && pending.empty()) return;
+            if (stop && stop_at == Clock::time_point{}) stop_at = Clock::now();
+            if (stop && Clock::now() - stop_at > std::chrono::seconds(30)) {
+                // the queue did not drain in time: report and leave
+                lk.unlock();
+                fail_all(GCL_E_INTERNAL);
+                return;
+            }
+            if (!inflight && pending.empty()) {
+                if (stop) return;
                 cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-            } else {
-                // a launch is running: poll again shortly (or at once when
-                // a slot is free and work arrives)
-                cv_work.wait_for(lk, std::chrono::microseconds(50));
+            } else if (got == 0) {
+                cv_work.wait_for(lk, std::chrono::microseconds(20));
             }
         }
     }
@@ -320,9 +354,7 @@ struct Server {
         std::lock_guard<std::mutex> lk(mu);
         *o = st;
         o->pending = pending.size();
-        uint64_t inf = 0;
-        for (auto& v : running) inf += v.size();
-        o->in_flight = inf;
+        o->in_flight = st.submitted - st.completed - pending.size();
         o->queued_bytes = queued_bytes;
         return GCL_OK;
     }
@@ -375,11 +407,8 @@ struct Server {
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                if (fatal != GCL_OK && pending.empty()) {
-                    bool idle = true;
-                    for (auto& v : running) idle = idle && v.empty();
-                    if (idle && done.empty() && got < count) return fail(fatal, "server stopped after a device error");
-                }
+                if (fatal != GCL_OK && st.completed == st.submitted && done.empty() && got < count)
+                    return fail(fatal, "server stopped after a device error");
             }
             if (k == 0 && next < count && !blocked) {
                 // sleep until the next arrival (busy-wait the last 200 us)
@@ -399,14 +428,12 @@ extern "C" {
 
 void gcl_server_config_default(gcl_server_config* c) {
     if (!c) return;
-    c->slots = 3;
-    c->max_graphs = 4096;
-    c->max_vertices = 1ll << 21;
-    c->max_nnz = 1ll << 24;
     c->max_n = 4096;
     c->max_graph_nnz = 1ll << 20;
     c->max_aut = 10000;
+    c->slots = 0;
     c->queue_bytes = 1ull << 30;
+    c->lifetime_ms = 0;
 }
 
 gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out) {

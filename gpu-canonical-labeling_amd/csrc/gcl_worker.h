@@ -128,6 +128,33 @@ __host__ __device__ inline SpecLayout make_spec_layout(uint32_t maxn, uint32_t s
     return L;
 }
 
+// Serving mode: requests published by the host while the kernel runs.  The
+// host side (fine-grained host memory, read with system-scope loads) holds
+// the queue and each request slot's graph; the claiming worker copies the
+// graph into the device arrays of its slot (vtx_off / row_ptr / col / color /
+// init_order of KParams, where graph 2s is slot s) and writes the labels, the
+// result record and the slot's done flag back to host memory.
+struct ServeParams {
+    const uint32_t* tail;    // requests published (host)
+    const uint32_t* stop;    // 1: no more requests will be published (host)
+    const uint32_t* ring;    // slot ids in publication order, ring_mask + 1 entries (host)
+    uint32_t ring_mask;
+    uint32_t slots;
+    const int32_t* h_n;      // per slot (host)
+    const int32_t* h_row;    // per slot: maxn + 1 local row offsets (host)
+    const int32_t* h_col;    // per slot: max_nnz neighbour ids (host)
+    const int32_t* h_color;  // per slot: maxn colours (host)
+    const int32_t* h_init;   // per slot: vertices in (colour, id) order (host)
+    const int32_t* h_colored;  // per slot: 1 when the graph has colours (host)
+    uint32_t* done;          // per slot: set to 1 once the results are written (host)
+    // device-memory mirror of the host queue: idle workers poll these; one
+    // worker at a time (the one that moves poll_at forward) reads the host
+    uint32_t* tail_seen;     // largest host tail read so far
+    uint32_t* stop_seen;     // 1 once the host's stop flag was read
+    unsigned long long* poll_at;  // wall-clock tick before which nobody reads the host
+    uint32_t poll_ticks;     // wall-clock ticks between host reads
+};
+
 struct KParams {
     int32_t num_items;
     const int32_t* items;       // graph ids processed by this launch
@@ -158,6 +185,7 @@ struct KParams {
     unsigned long long* board_mask;  // bit r set while record r is published
     unsigned int* pending;      // running graphs that may still publish
     uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
+    ServeParams sv;             // serving mode when sv.tail != nullptr
 };
 
 // ------------------------------------------------------------ wave helpers
@@ -1998,6 +2026,37 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.steps = 0;
 }
 
+// Searches graph g (vertices vtx_off[g] .. vtx_off[g+1]-1) and writes its
+// labels and result record.
+template <class Hot>
+__device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
+    const int l = lane_id();
+    const int64_t vbase = P.vtx_off[g];
+    W.n = (uint32_t)(P.vtx_off[g + 1] - vbase);
+    W.nw = (W.n + 63) / 64;
+    W.rowp = P.row_ptr + vbase;
+    W.col = P.col + W.rowp[0];
+    W.nnz = (uint64_t)(W.rowp[W.n] - W.rowp[0]);
+    W.H.bind(W.rowp, W.n, nullptr, P.maxn);
+    W.H.sync();
+    const uint64_t t0 = wall_clock64();
+    W.run(P, (uint32_t)g, vbase);
+    const uint64_t t1 = wall_clock64();
+    for (uint32_t i = (uint32_t)l; i < W.n; i += kWave) P.out_label[vbase + i] = (int32_t)W.best_label[i];
+    if (l == 0) {
+        DeviceGraphResult res;
+        res.nodes = W.nodes;
+        res.automorphisms = (uint64_t)W.num_aut;
+        res.splitter_steps = W.steps;
+        res.csr_bytes = W.csr_bytes;
+        res.device_ns = t1 - t0;
+        res.status = W.status;
+        res.pad = 0;
+        P.out[g] = res;
+    }
+    group_sync();
+}
+
 template <class Hot>
 __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
@@ -2012,31 +2071,7 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             if (l == 0) item = atomicAdd(P.next_item, 1u);
             item = __shfl(item, 0, kWave);
             if (item < (uint32_t)P.num_items) {
-                const int32_t g = P.items[item];
-                const int64_t vbase = P.vtx_off[g];
-                W.n = (uint32_t)(P.vtx_off[g + 1] - vbase);
-                W.nw = (W.n + 63) / 64;
-                W.rowp = P.row_ptr + vbase;
-                W.col = P.col + W.rowp[0];
-                W.nnz = (uint64_t)(W.rowp[W.n] - W.rowp[0]);
-                W.H.bind(W.rowp, W.n, nullptr, P.maxn);
-                W.H.sync();
-                const uint64_t t0 = wall_clock64();
-                W.run(P, (uint32_t)g, vbase);
-                const uint64_t t1 = wall_clock64();
-                for (uint32_t i = (uint32_t)l; i < W.n; i += kWave) P.out_label[vbase + i] = (int32_t)W.best_label[i];
-                if (l == 0) {
-                    DeviceGraphResult res;
-                    res.nodes = W.nodes;
-                    res.automorphisms = (uint64_t)W.num_aut;
-                    res.splitter_steps = W.steps;
-                    res.csr_bytes = W.csr_bytes;
-                    res.device_ns = t1 - t0;
-                    res.status = W.status;
-                    res.pad = 0;
-                    P.out[g] = res;
-                }
-                group_sync();
+                process_graph(W, P, P.items[item]);
                 continue;
             }
             graphs_left = false;
@@ -2054,6 +2089,132 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     }
 }
 
+// Poll of host-coherent memory.  Relaxed: an acquire at system scope would
+// invalidate the caches that the searches of the other workers depend on, and
+// the request data read after a poll is itself uncached host memory, read
+// only once the poll's value is known.
+__device__ __forceinline__ uint32_t sys_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Serving loop: claims published requests of this class until the host
+// stops the queue (or the worker's deadline passes).  Root children of large
+// graphs published by other workers are helped first, as in worker_loop.
+template <class Hot>
+__device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
+    const int l = lane_id();
+    const ServeParams& S = P.sv;
+    uint32_t idle = 1;
+    while (true) {
+        if (P.spec_min_n && help_once(W, P) > 0) {
+            idle = 1;
+            continue;
+        }
+        uint32_t slot = 0xFFFFFFFFu, stop = 0, retry = 0;
+        if (l == 0) {
+            const uint32_t h = __hip_atomic_load(P.next_item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t t = __hip_atomic_load(S.tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (h >= t) {
+                // the queue looks empty: refresh the mirror from the host if
+                // nobody did within poll_ticks.  One worker per period reads
+                // host memory, and the lease's CAS is only tried once the
+                // period has passed: thousands of idle workers polling over
+                // PCIe, or all trying the CAS, would hold up the memory
+                // requests of the workers that are searching.
+                const uint64_t now = wall_clock64();
+                unsigned long long due = __hip_atomic_load(S.poll_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (now >= due && __hip_atomic_compare_exchange_strong(S.poll_at, &due, now + S.poll_ticks,
+                                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_AGENT)) {
+                    const uint32_t hs = sys_load(S.stop);
+                    // read after the stop flag: the host publishes before it
+                    // stops, so a stop read here comes with the final tail
+                    const uint32_t ht = sys_load(S.tail);
+                    if (ht > t) {
+                        __hip_atomic_fetch_max(S.tail_seen, ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        t = ht;
+                    }
+                    if (hs) __hip_atomic_store(S.stop_seen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            if (h < t) {
+                uint32_t expect = h;
+                if (__hip_atomic_compare_exchange_strong(P.next_item, &expect, h + 1, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    slot = sys_load(&S.ring[h & S.ring_mask]);
+                else
+                    retry = 1;  // another worker took it: try again at once
+            } else if (__hip_atomic_load(S.stop_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+                // the tail stored before the stop flag is visible now
+                stop = __hip_atomic_load(S.tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= h ? 1u : 0u;
+                retry = 1u - stop;
+            }
+        }
+        slot = uni(readlane(slot, 0));
+        if (uni(readlane(retry, 0))) continue;
+        if (slot != 0xFFFFFFFFu && slot < S.slots) {
+            // copy the request's graph into the slot's device arrays (the
+            // host memory is read with vector loads: the scalar cache is not
+            // kept coherent with host writes while the kernel runs)
+            const uint64_t n1 = (uint64_t)P.maxn + 1;
+            const int32_t* hr = S.h_row + (uint64_t)slot * n1;
+            const uint32_t n = uni((uint32_t)__hip_atomic_load(&S.h_n[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+            const uint32_t nnz = n <= P.maxn
+                                     ? uni((uint32_t)__hip_atomic_load(&hr[n], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                                     : 0xFFFFFFFFu;
+            if (n > P.maxn || nnz > P.max_nnz) {  // not a request this kernel can hold
+                if (l == 0) {
+                    DeviceGraphResult res{};
+                    res.status = GCL_E_INVALID;
+                    P.out[2 * slot] = res;
+                    __threadfence_system();
+                    __hip_atomic_store(&S.done[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                continue;
+            }
+            const int64_t vb = (int64_t)((uint64_t)slot * n1);
+            const int64_t cb = (int64_t)((uint64_t)slot * P.max_nnz);
+            int64_t* drow = const_cast<int64_t*>(P.row_ptr) + vb;
+            for (uint32_t i = (uint32_t)l; i <= n; i += kWave) drow[i] = cb + hr[i];
+            const int32_t* hc = S.h_col + (uint64_t)slot * P.max_nnz;
+            int32_t* dcol = const_cast<int32_t*>(P.col) + cb;
+            for (uint32_t e = (uint32_t)l; e < nnz; e += kWave) dcol[e] = hc[e];
+            const bool colored =
+                uni((uint32_t)__hip_atomic_load(&S.h_colored[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) != 0u;
+            int32_t* dcolor = const_cast<int32_t*>(P.color) + vb;
+            int32_t* dinit = const_cast<int32_t*>(P.init_order) + vb;
+            const int32_t* hcol = S.h_color + (uint64_t)slot * P.maxn;
+            const int32_t* hini = S.h_init + (uint64_t)slot * P.maxn;
+            for (uint32_t i = (uint32_t)l; i < n; i += kWave) {
+                dcolor[i] = colored ? hcol[i] : 0;
+                dinit[i] = colored ? hini[i] : (int32_t)i;
+            }
+            int64_t* dvtx = const_cast<int64_t*>(P.vtx_off);
+            if (l == 0) {
+                dvtx[2 * slot] = vb;
+                dvtx[2 * slot + 1] = vb + (int64_t)n;
+            }
+            __threadfence();
+            group_sync();
+            process_graph(W, P, (int32_t)(2 * slot));
+            // labels and record reach host memory before the done flag
+            __threadfence_system();
+            group_sync();
+            if (l == 0) __hip_atomic_store(&S.done[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            idle = 1;
+            continue;
+        }
+        if (uni(readlane(stop, 0)) || wall_clock64() > W.deadline) break;
+        // nothing to do: back off (up to ~0.9 ms).  With thousands of idle
+        // workers a request or a published root child is still seen within
+        // microseconds, and their polls (loads, and the lease's CAS on one
+        // L2 line) stay too rare to delay the memory requests of the
+        // workers that are searching.
+        for (uint32_t i = 0; i < idle; ++i) __builtin_amdgcn_s_sleep(127);
+        idle = idle < 256 ? idle * 2 : 256;
+    }
+}
+
 // The worker object lives in LDS: its uniform fields are reloaded where used
 // instead of occupying registers for the whole persistent loop.
 template <int MAXN>
@@ -2063,6 +2224,16 @@ __global__ __launch_bounds__(kWave) void search_kernel_lds(KParams P) {
     const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false);
     bind_scratch(W, P, base, L);
     worker_loop(W, P);
+}
+
+// Serving kernel of the same size class: runs until the host stops its queue.
+template <int MAXN>
+__global__ __launch_bounds__(kWave) void serve_kernel_lds(KParams P) {
+    __shared__ Worker<HotL<MAXN>> W;
+    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false);
+    bind_scratch(W, P, base, L);
+    serve_loop(W, P);
 }
 
 __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
@@ -2076,6 +2247,19 @@ __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
     W.H.n1 = hot_stride(P.maxn);
     W.H.nw1 = (uint64_t)P.maxn / 64 + 2;
     worker_loop(W, P);
+}
+
+__global__ __launch_bounds__(kWave) void serve_kernel_global(KParams P) {
+    __shared__ Worker<HotG> W;
+    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true);
+    bind_scratch(W, P, base, L);
+    W.H.h16 = (ix_t*)(base + L.hot16);
+    W.H.h32 = (uint32_t*)(base + L.hot32);
+    W.H.h64 = (uint64_t*)(base + L.hot64);
+    W.H.n1 = hot_stride(P.maxn);
+    W.H.nw1 = (uint64_t)P.maxn / 64 + 2;
+    serve_loop(W, P);
 }
 
 }  // namespace dev

@@ -67,14 +67,14 @@ class _ClassStats(C.Structure):
 
 
 class _ServerConfig(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("max_graphs", C.c_int32), ("max_vertices", C.c_int64),
-                ("max_nnz", C.c_int64), ("max_n", C.c_int32), ("max_graph_nnz", C.c_int64),
-                ("max_aut", C.c_uint32), ("queue_bytes", C.c_uint64)]
+    _fields_ = [("max_n", C.c_int32), ("max_graph_nnz", C.c_int64), ("max_aut", C.c_uint32), ("slots", C.c_int32),
+                ("queue_bytes", C.c_uint64), ("lifetime_ms", C.c_int32)]
 
 
 class _ServerResult(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("verdict", C.c_int32), ("n", C.c_int32),
-                ("launch_graphs", C.c_int32), ("latency_ms", C.c_double), ("nodes", C.c_uint64)]
+                ("pad", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("nodes", C.c_uint64),
+                ("device_ms", C.c_double)]
 
 
 class _ServerStats(C.Structure):

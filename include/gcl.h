@@ -255,27 +255,29 @@ void gcl_owned_batch_free(gcl_owned_batch* b);
 /* ---------------------------------------------------------------- server */
 /*
  * Request server: canonicalize and isomorphism requests submitted as they
- * arrive, run with continuous batching on device 0.  Every device buffer
- * (CSR arena, per-class worker scratch, speculation records) and the pinned
- * host staging are allocated once at create: `slots` launch contexts, each
- * with the per-launch capacity below.  A scheduler thread packs queued
- * requests (in arrival order) into any free slot while the other slots' work
- * runs on the device, and completes a slot's requests when its launch ends.
- * Admission is by a footprint estimated from n and m: a request whose graph
- * exceeds a slot's capacity is refused with GCL_E_CAPACITY, and one that
- * would take the queued footprint past queue_bytes with GCL_E_BUSY (retry
- * later).  No counterpart in the reference (its only entry point is the
- * per-graph Nauty_Parallel call, src/nauty_parallel.h:9-13).
+ * arrive, run with continuous batching on device 0.  At create, one
+ * persistent search kernel per size class is started, each on a fixed share
+ * of the chip, and every buffer is allocated once: per class, request slots
+ * in host-coherent memory (graph in, labels and result out), the device
+ * arrays a worker copies a claimed request into, worker scratch and
+ * speculation records.  A scheduler thread writes each request's graphs into
+ * free slots of their size classes and publishes them to the class queues,
+ * in arrival order; idle workers claim them while the others keep searching,
+ * and a request completes as soon as its own graphs are done.  Admission is
+ * by a footprint estimated from n and m: a graph larger than every class
+ * (max_n, max_graph_nnz) is refused with GCL_E_CAPACITY, a request that would
+ * take the queued footprint past queue_bytes with GCL_E_BUSY (retry later);
+ * requests wait in the host queue while their class has no free slot.  No
+ * counterpart in the reference (its only entry point is the per-graph
+ * Nauty_Parallel call, src/nauty_parallel.h:9-13).
  */
 typedef struct gcl_server_config {
-    int32_t slots;             /* launch contexts (>= 1); default 3 */
-    int32_t max_graphs;        /* graphs per launch; default 4096 */
-    int64_t max_vertices;      /* sum of n per launch; default 2^21 */
-    int64_t max_nnz;           /* sum of directed edges per launch; default 2^24 */
     int32_t max_n;             /* largest graph accepted; default 4096 */
     int64_t max_graph_nnz;     /* directed edges of the largest graph; default 2^20 */
     uint32_t max_aut;          /* automorphism ring per search (>= 1); default 10000 */
+    int32_t slots;             /* request slots per size class; 0 (default): twice its workers */
     uint64_t queue_bytes;      /* footprint budget of queued + running requests; default 2^30 */
+    int32_t lifetime_ms;       /* serving kernels stop after this long; 0 (default): no limit */
 } gcl_server_config;
 
 /* Fills the defaults listed above. */
@@ -288,13 +290,17 @@ typedef struct gcl_server_result {
     int32_t status;            /* gcl_status of the request */
     int32_t verdict;           /* pair: 1 isomorphic, 0 not; canonicalize: -1 */
     int32_t n;                 /* vertices of the first graph */
-    int32_t launch_graphs;     /* graphs in the launch that ran it */
+    int32_t pad;
     double latency_ms;         /* submit (or scheduled arrival) to completion */
+    double queue_ms;           /* submit to publication on the device queue */
     uint64_t nodes;            /* search-tree nodes (both graphs of a pair) */
+    double device_ms;          /* search time on the device (both graphs of a pair) */
 } gcl_server_result;
 
 typedef struct gcl_server_stats {
-    uint64_t submitted, completed, refused_busy, launches, graphs_launched;
+    uint64_t submitted, completed, refused_busy;
+    uint64_t launches;         /* serving kernels (one per size class) */
+    uint64_t graphs_launched;  /* graphs published to the device queues */
     uint64_t pending, in_flight, queued_bytes;
 } gcl_server_stats;
 

@@ -60,8 +60,8 @@ def test_invalid_arguments_rejected_without_device():
 
 def test_server_config_defaults_and_no_device():
     c = gcl.server_config()
-    assert (c.slots, c.max_graphs, c.max_n, c.max_aut) == (3, 4096, 4096, 10000)
-    assert c.max_vertices >= 2 * c.max_n and c.queue_bytes > 0
+    assert (c.slots, c.max_n, c.max_aut, c.lifetime_ms) == (0, 4096, 10000, 0)
+    assert c.max_graph_nnz > 0 and c.queue_bytes > 0
     if gcl.device_count() == 0:
         try:
             gcl.Server()

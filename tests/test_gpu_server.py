@@ -36,16 +36,16 @@ def _mix(count, seed):
 
 def test_server_same_alone_and_in_mix():
     graphs = _mix(90, 5)
-    srv = gcl.Server(slots=3, max_n=1024, max_aut=64)
+    srv = gcl.Server(max_n=1024, max_aut=64, lifetime_ms=60000)
     try:
         _log("alone")
         alone = {}
         for i in (0, 7, 33, 60, 89):
             t = srv.submit(graphs[i])
-            alone[i] = srv.wait_all([t])[t]["labels"].copy()
+            alone[i] = srv.wait_all([t], timeout_s=60)[t]["labels"].copy()
         _log("mix")
         tickets = [srv.submit(g) for g in graphs]
-        res = srv.wait_all(tickets)
+        res = srv.wait_all(tickets, timeout_s=60)
         for i, t in enumerate(tickets):
             r = res[t]
             assert r["status"] == 0, (i, r)
@@ -56,21 +56,21 @@ def test_server_same_alone_and_in_mix():
                 assert np.array_equal(r["labels"], alone[i]), i
         st = srv.stats()
         assert st["completed"] >= 95 and st["pending"] == 0 and st["in_flight"] == 0
-        assert st["launches"] >= 2
+        assert st["launches"] == 1  # one serving kernel for n <= 1024
     finally:
         srv.close()
 
 
 def test_server_pair_verdicts_match_oracle():
     base = _mix(24, 9)
-    srv = gcl.Server(slots=2, max_n=1024)
+    srv = gcl.Server(max_n=1024, lifetime_ms=60000)
     try:
         pairs = []
         for i, g in enumerate(base):
             other = gcl.permute(g, 500 + i) if i % 2 == 0 else gcl.edge_swap(g, 700 + i)
             pairs.append((g, other))
         tickets = [srv.submit(a, b) for a, b in pairs]
-        res = srv.wait_all(tickets)
+        res = srv.wait_all(tickets, timeout_s=60)
         for (a, b), t in zip(pairs, tickets):
             want = np.array_equal(oracle.canon_graph(a)["keys"], oracle.canon_graph(b)["keys"]) if a.m == b.m else False
             assert res[t]["status"] == 0
@@ -80,7 +80,7 @@ def test_server_pair_verdicts_match_oracle():
 
 
 def test_server_admission():
-    srv = gcl.Server(slots=1, max_n=256, max_graph_nnz=4096, queue_bytes=64 * 1024)
+    srv = gcl.Server(max_n=256, max_graph_nnz=4096, queue_bytes=64 * 1024, lifetime_ms=60000)
     try:
         big = gcl.random_regular(300, 3, 1)
         with pytest.raises(gcl.GclError) as e:
@@ -96,7 +96,7 @@ def test_server_admission():
                 assert e.status == -8  # busy
                 busy += 1
         assert busy > 0 and tickets
-        res = srv.wait_all(tickets)
+        res = srv.wait_all(tickets, timeout_s=60)
         assert all(r["status"] == 0 for r in res.values())
         assert srv.stats()["refused_busy"] == busy
     finally:
@@ -110,7 +110,7 @@ def test_server_replay_poisson():
     arrival = np.cumsum(rs.exponential(1.0 / 4000.0, size=len(graphs)))
     pair_with = np.full(len(graphs), -1, np.int32)
     pair_with[::4] = np.arange(1, len(graphs), 4)[: len(pair_with[::4])]
-    srv = gcl.Server(slots=3, max_n=1024)
+    srv = gcl.Server(max_n=1024, lifetime_ms=60000)
     try:
         _log("replay")
         r = srv.replay(batch, arrival, pair_with)
