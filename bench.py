@@ -118,9 +118,11 @@ def served(args):
     srv = gcl.Server(max_n=1024, max_aut=10000, lifetime_ms=600000)
     try:
         srv.replay(batch, np.zeros(min(R, 512)), pair_with[:min(R, 512)])  # warm-up
+        s0 = srv.stats()
         t1 = time.perf_counter()
         r1 = srv.replay(batch, np.zeros(R), pair_with)
         cap_s = time.perf_counter() - t1
+        s1 = srv.stats()
         if np.any(r1["status"] != 0):
             raise SystemExit("bench.py: served requests failed")
         cap = labelings / cap_s
@@ -143,6 +145,10 @@ def served(args):
                            "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
             "poisson_rate_req_s": round(rate, 1), "poisson_wall_s": round(lat_s, 3),
             "queue_ms_p50": round(float(np.percentile(r2["queue_ms"], 50)), 3),
+            "device_ms": {"p50": round(float(np.percentile(r2["device_ms"], 50)), 3),
+                          "p99": round(float(np.percentile(r2["device_ms"], 99)), 3)},
+            "capacity_scheduler_ms": {k: round(s1["sched_" + k + "_ms"] - s0["sched_" + k + "_ms"], 1)
+                                      for k in ("publish", "collect")},
             "serving_kernels": int(st["launches"]),
             "workload": "same size mix (n log-uniform [16,1024]); half canonicalize, half pair tests "
                         "(permuted copy or one-swap variant, 50/50); persistent serving kernels, one per size "

@@ -67,7 +67,6 @@ struct ServeSlotView {  // host-coherent arrays of one request slot
     int32_t* colored;
     const int32_t* label;
     const DeviceGraphResult* out;
-    uint32_t* done;     // set by the device once label and out are written
 };
 struct ServeEngine;
 gcl_status serve_create(const ServeConfig& c, ServeEngine** out);
@@ -77,6 +76,9 @@ ServeClassInfo serve_class_info(const ServeEngine* s, int k);
 int serve_class_of(const ServeEngine* s, int64_t n, int64_t nnz);  // -1: too large
 ServeSlotView serve_slot(ServeEngine* s, int k, uint32_t slot);
 void serve_publish(ServeEngine* s, int k, uint32_t slot);  // slot data written before
+// Slots of class k completed since the last call, in completion order (up to
+// cap); their labels and result records are readable.  One caller at a time.
+int serve_completions(ServeEngine* s, int k, uint32_t* out, int cap);
 void serve_stop(ServeEngine* s);
 gcl_status serve_wait(ServeEngine* s);
 gcl_status serve_check(ServeEngine* s);  // GCL_E_DEVICE once a serving kernel has failed

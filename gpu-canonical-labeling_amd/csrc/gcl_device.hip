@@ -530,10 +530,11 @@ struct ServeEngine {
         uint32_t maxn = 0, slots = 0, ring = 0;
         uint64_t max_nnz = 0;
         int64_t grid = 0;
-        HostBuf q, ringb, hn, hrow, hcol, hcolor, hinit, hcolored, done, out, label;
+        HostBuf q, ringb, hn, hrow, hcol, hcolor, hinit, hcolored, cq, out, label;
         DevBuf vtx, row, col, color, init, scratch, spec, board, ctr;
         hipStream_t stream = nullptr;
-        uint32_t tail = 0;  // host copy of the published count
+        uint32_t tail = 0;     // host copy of the published count
+        uint32_t cq_head = 0;  // completions consumed
     };
     std::vector<Cls> classes;
     bool running = false;
@@ -599,13 +600,13 @@ struct ServeEngine {
             uint32_t* q;
             uint32_t* ringp;
             int32_t *hn, *hrow, *hcol, *hcolor, *hinit, *hcolored, *label;
-            uint32_t* done;
+            unsigned long long* cq;
             DeviceGraphResult* out;
             gcl_status st;
             if ((st = hmalloc(C.q, 64, &q)) || (st = hmalloc(C.ringb, C.ring, &ringp)) || (st = hmalloc(C.hn, S, &hn)) ||
                 (st = hmalloc(C.hrow, S * n1, &hrow)) || (st = hmalloc(C.hcol, S * C.max_nnz, &hcol)) ||
                 (st = hmalloc(C.hcolor, S * C.maxn, &hcolor)) || (st = hmalloc(C.hinit, S * C.maxn, &hinit)) ||
-                (st = hmalloc(C.hcolored, S, &hcolored)) || (st = hmalloc(C.done, S, &done)) ||
+                (st = hmalloc(C.hcolored, S, &hcolored)) || (st = hmalloc(C.cq, C.ring, &cq)) ||
                 (st = hmalloc(C.out, 2 * S, &out)) || (st = hmalloc(C.label, S * n1, &label)))
                 return st;
             GCL_HIP(hipMalloc(&C.vtx.p, sizeof(int64_t) * 2 * S));
@@ -615,7 +616,8 @@ struct ServeEngine {
             GCL_HIP(hipMalloc(&C.init.p, sizeof(int32_t) * S * C.maxn));
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)grid)));
             // work head, pending counter (never 0 while serving), the host
-            // queue's mirror (tail, stop) and the next host-poll tick
+            // queue's mirror (tail, stop), the next host-poll tick and the
+            // completion count
             unsigned int ctr_h[8] = {0u, 0x40000000u, 0u, 0u, 0u, 0u, 0u, 0u};
             GCL_HIP(hipMalloc(&C.ctr.p, sizeof(ctr_h)));
             GCL_HIP(hipMemcpy(C.ctr.p, ctr_h, sizeof(ctr_h), hipMemcpyHostToDevice));
@@ -687,8 +689,10 @@ struct ServeEngine {
             V.h_init = (const int32_t*)dp;
             GCL_HIP(hipHostGetDevicePointer(&dp, C.hcolored.p, 0));
             V.h_colored = (const int32_t*)dp;
-            GCL_HIP(hipHostGetDevicePointer(&dp, C.done.p, 0));
-            V.done = (uint32_t*)dp;
+            GCL_HIP(hipHostGetDevicePointer(&dp, C.cq.p, 0));
+            V.cq = (unsigned long long*)dp;
+            V.cq_mask = C.ring - 1;  // C.ring >= slots: a slot has one completion pending at most
+            V.cq_count = (unsigned int*)C.ctr.p + 6;
             V.tail_seen = (uint32_t*)C.ctr.p + 2;
             V.stop_seen = (uint32_t*)C.ctr.p + 3;
             V.poll_at = (unsigned long long*)((uint32_t*)C.ctr.p + 4);
@@ -771,8 +775,19 @@ ServeSlotView serve_slot(ServeEngine* s, int k, uint32_t slot) {
     v.colored = (int32_t*)c.hcolored.p + slot;
     v.label = (const int32_t*)c.label.p + slot * n1;
     v.out = (const DeviceGraphResult*)c.out.p + 2 * (size_t)slot;
-    v.done = (uint32_t*)c.done.p + slot;
     return v;
+}
+int serve_completions(ServeEngine* s, int k, uint32_t* out, int cap) {
+    auto& c = s->classes[(size_t)k];
+    const unsigned long long* cq = (const unsigned long long*)c.cq.p;
+    int got = 0;
+    while (got < cap) {
+        const unsigned long long e = __atomic_load_n(&cq[c.cq_head & (c.ring - 1)], __ATOMIC_ACQUIRE);
+        if ((uint32_t)(e >> 32) != c.cq_head + 1u) break;  // not written yet
+        out[got++] = (uint32_t)e;
+        ++c.cq_head;
+    }
+    return got;
 }
 void serve_publish(ServeEngine* s, int k, uint32_t slot) {
     auto& c = s->classes[(size_t)k];

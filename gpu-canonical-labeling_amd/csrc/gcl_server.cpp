@@ -2,6 +2,7 @@
 // continuous batching over fixed launch slots, completion and load replay.
 // The device side of a slot (buffers, kernels) is in gcl_device.hip.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -57,8 +58,9 @@ struct Server {
     struct ClassState {
         std::vector<uint32_t> free_slots;
         std::vector<Slot> slots;
-        std::vector<uint32_t> busy;  // slots in flight
+        size_t busy = 0;  // slots in flight
     };
+    std::vector<uint32_t> cbuf = std::vector<uint32_t>(1024);  // completed slot ids
     std::vector<ClassState> cls;
     struct Live {
         int remaining = 0;
@@ -70,6 +72,7 @@ struct Server {
     };
     std::unordered_map<Request*, Live> live;
     uint64_t ticks_per_ms = 100000;
+    std::atomic<uint64_t> publish_ns{0}, collect_ns{0};  // scheduler thread time
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
@@ -174,10 +177,9 @@ struct Server {
         } else {
             *v.colored = 0;
         }
-        __atomic_store_n(v.done, 0u, __ATOMIC_RELAXED);
         __atomic_store_n(v.n, g.n, __ATOMIC_RELAXED);
         cls[(size_t)k].slots[s] = Slot{q, part};
-        cls[(size_t)k].busy.push_back(s);
+        ++cls[(size_t)k].busy;
         serve_publish(eng, k, s);
     }
 
@@ -246,31 +248,32 @@ struct Server {
         ++st.completed;
     }
 
-    // Collects slots whose done flag the device has set; returns how many.
+    // Collects the slots the device reported done; returns how many.
     size_t collect() {
         size_t got = 0;
         std::vector<Request*> finished;
         for (size_t k = 0; k < cls.size(); ++k) {
             auto& C = cls[k];
-            for (size_t i = 0; i < C.busy.size();) {
-                const uint32_t s = C.busy[i];
-                ServeSlotView v = serve_slot(eng, (int)k, s);
-                if (__atomic_load_n(v.done, __ATOMIC_ACQUIRE) == 0u) {
-                    ++i;
-                    continue;
+            for (;;) {
+                const int m = C.busy > 0 ? serve_completions(eng, (int)k, cbuf.data(), (int)cbuf.size()) : 0;
+                if (m == 0) break;
+                for (int i = 0; i < m; ++i) {
+                    const uint32_t s = cbuf[(size_t)i];
+                    if (s >= C.slots.size() || C.slots[s].req == nullptr) continue;  // not a slot in flight
+                    ServeSlotView v = serve_slot(eng, (int)k, s);
+                    Slot sl = C.slots[s];
+                    C.slots[s] = Slot{};
+                    Live& L = live[sl.req];
+                    const DeviceGraphResult& d = *v.out;
+                    if (d.status != 0 && L.status == 0) L.status = d.status;
+                    L.nodes += d.nodes;
+                    L.dev_ticks += d.device_ns;
+                    L.lab[sl.part].assign(v.label, v.label + sl.req->g[sl.part].n);
+                    if (--L.remaining == 0) finished.push_back(sl.req);
+                    --C.busy;
+                    C.free_slots.push_back(s);
+                    ++got;
                 }
-                Slot sl = C.slots[s];
-                Live& L = live[sl.req];
-                const DeviceGraphResult& d = *v.out;
-                if (d.status != 0 && L.status == 0) L.status = d.status;
-                L.nodes += d.nodes;
-                L.dev_ticks += d.device_ns;
-                L.lab[sl.part].assign(v.label, v.label + sl.req->g[sl.part].n);
-                if (--L.remaining == 0) finished.push_back(sl.req);
-                C.busy[i] = C.busy.back();
-                C.busy.pop_back();
-                C.free_slots.push_back(s);
-                ++got;
             }
         }
         for (Request* q : finished) {
@@ -298,7 +301,10 @@ struct Server {
         };
         for (auto& kv : live) drop(kv.first);
         live.clear();
-        for (auto& C : cls) C.busy.clear();
+        for (auto& C : cls) {
+            C.busy = 0;
+            for (auto& sl : C.slots) sl = Slot{};
+        }
         while (!pending.empty()) {
             drop(pending.front());
             pending.pop_front();
@@ -310,10 +316,15 @@ struct Server {
         auto last_check = Clock::now();
         Clock::time_point stop_at{};
         for (;;) {
+            const auto c0 = Clock::now();
             const size_t got = collect();
+            const auto c1 = Clock::now();
             dispatch();
+            const auto c2 = Clock::now();
+            collect_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(c1 - c0).count();
+            publish_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(c2 - c1).count();
             bool inflight = false;
-            for (auto& C : cls) inflight = inflight || !C.busy.empty();
+            for (auto& C : cls) inflight = inflight || C.busy > 0;
             if (inflight && Clock::now() - last_check > std::chrono::milliseconds(20)) {
                 last_check = Clock::now();
                 if (serve_check(eng) != GCL_OK) {
@@ -356,6 +367,8 @@ struct Server {
         o->pending = pending.size();
         o->in_flight = st.submitted - st.completed - pending.size();
         o->queued_bytes = queued_bytes;
+        o->sched_publish_ms = (double)publish_ns.load() / 1e6;
+        o->sched_collect_ms = (double)collect_ns.load() / 1e6;
         return GCL_OK;
     }
 

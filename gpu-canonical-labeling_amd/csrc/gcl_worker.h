@@ -146,7 +146,11 @@ struct ServeParams {
     const int32_t* h_color;  // per slot: maxn colours (host)
     const int32_t* h_init;   // per slot: vertices in (colour, id) order (host)
     const int32_t* h_colored;  // per slot: 1 when the graph has colours (host)
-    uint32_t* done;          // per slot: set to 1 once the results are written (host)
+    // completion queue (host): entry c = ((c + 1) << 32) | slot, written once
+    // the slot's labels and result record are in host memory
+    unsigned long long* cq;
+    uint32_t cq_mask;        // entries - 1 (at least the slot count)
+    unsigned int* cq_count;  // completions so far (device memory)
     // device-memory mirror of the host queue: idle workers poll these; one
     // worker at a time (the one that moves poll_at forward) reads the host
     uint32_t* tail_seen;     // largest host tail read so far
@@ -2097,6 +2101,16 @@ __device__ __forceinline__ uint32_t sys_load(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Lane 0: reports slot as done.  Its labels and result record were written
+// and made visible to the host (__threadfence_system) before.  The host reads
+// entries in order; an entry whose sequence number is not the next one it
+// expects has not been written yet.
+__device__ __forceinline__ void serve_complete(const ServeParams& S, uint32_t slot) {
+    const uint32_t c = atomicAdd(S.cq_count, 1u);
+    __hip_atomic_store(&S.cq[c & S.cq_mask], ((unsigned long long)(c + 1u) << 32) | slot, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Serving loop: claims published requests of this class until the host
 // stops the queue (or the worker's deadline passes).  Root children of large
 // graphs published by other workers are helped first, as in worker_loop.
@@ -2168,7 +2182,7 @@ __device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
                     res.status = GCL_E_INVALID;
                     P.out[2 * slot] = res;
                     __threadfence_system();
-                    __hip_atomic_store(&S.done[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    serve_complete(S, slot);
                 }
                 continue;
             }
@@ -2200,7 +2214,7 @@ __device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
             // labels and record reach host memory before the done flag
             __threadfence_system();
             group_sync();
-            if (l == 0) __hip_atomic_store(&S.done[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (l == 0) serve_complete(S, slot);
             idle = 1;
             continue;
         }

@@ -79,7 +79,8 @@ class _ServerResult(C.Structure):
 
 class _ServerStats(C.Structure):
     _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "refused_busy", "launches", "graphs_launched",
-                                           "pending", "in_flight", "queued_bytes")]
+                                           "pending", "in_flight", "queued_bytes")] + \
+               [("sched_publish_ms", C.c_double), ("sched_collect_ms", C.c_double)]
 
 
 _lib = None

@@ -302,6 +302,8 @@ typedef struct gcl_server_stats {
     uint64_t launches;         /* serving kernels (one per size class) */
     uint64_t graphs_launched;  /* graphs published to the device queues */
     uint64_t pending, in_flight, queued_bytes;
+    double sched_publish_ms;   /* scheduler thread: writing requests into slots and publishing them */
+    double sched_collect_ms;   /* scheduler thread: reading results, pair verdicts */
 } gcl_server_stats;
 
 gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out);
