@@ -121,3 +121,31 @@ def test_server_replay_poisson():
         assert st["completed"] == len(graphs)
     finally:
         srv.close()
+
+
+def test_server_golden_graphs_and_pairs(golden):
+    """Every golden graph (one vertex, no edges, coloured, ...) through the
+    server gives the reference's `output` bytes; golden pairs its verdicts."""
+    from conftest import golden_graph, golden_output
+    srv = gcl.Server(max_n=1024, max_aut=10000, lifetime_ms=60000)
+    try:
+        singles = [e for e in golden["singles"] if e["n"] <= 1024]
+        _log(f"golden singles {len(singles)}")
+        gs = [golden_graph(e["name"]) for e in singles]
+        tickets = [srv.submit(g) for g in gs]
+        res = srv.wait_all(tickets, timeout_s=60)
+        for e, g, t in zip(singles, gs, tickets):
+            r = res[t]
+            assert r["status"] == 0, (e["name"], r)
+            keys = gcl.canonical_keys(g, r["labels"])
+            assert gcl.format_adjacency(keys, g.n).encode() == golden_output(e), e["name"]
+        _log(f"golden pairs {len(golden['pairs'])}")
+        pairs = [(golden_graph(p["a"]), golden_graph(p["b"])) for p in golden["pairs"]]
+        tickets = [srv.submit(a, b) for a, b in pairs]
+        res = srv.wait_all(tickets, timeout_s=60)
+        for p, t in zip(golden["pairs"], tickets):
+            r = res[t]
+            assert r["status"] == 0, (p, r)
+            assert ("ISOMORPHIC" if r["verdict"] == 1 else "NOT ISOMORPHIC") == p["verdict"], p
+    finally:
+        srv.close()
