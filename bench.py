@@ -54,7 +54,9 @@ def traffic_from_profile(kernel, graphs):
         t = json.load(f)
     if t.get("graphs") != graphs:
         return None
-    k = t.get("kernels", {}).get(kernel)
+    # the profile names kernels with their parameter list, "name(KParams)"
+    by_name = {n.split("(")[0]: v for n, v in t.get("kernels", {}).items()}
+    k = by_name.get(kernel)
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
@@ -168,7 +170,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--skip-baselines", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=6000,
+    ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the served measurement (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -269,7 +271,7 @@ def main():
             out["served"] = served(args)
             out["served_vs_plain"] = round(out["served"]["value"] / plain_rate, 2)
             out["p99_latency_ms"] = out["served"]["latency_ms"]["p99"]
-        out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 512)
+        out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 256)
     print(json.dumps(out), flush=True)
 
 

@@ -307,6 +307,7 @@ struct DeviceBatch {
             GCL_HIP(hipEventCreate(&C.t1));
             C.graphs = items_h[cls];
             KParams& P = C.P;
+            memset(&P, 0, sizeof(P));
             P.num_items = (int32_t)cnt;
             P.items = (const int32_t*)items.p + item_off;
             P.vtx_off = (const int64_t*)vtx.p;
