@@ -1620,19 +1620,27 @@ struct Worker {
     __device__ SpecHeader* spec_header() { return (SpecHeader*)(spec_rec + SL.header); }
     __device__ uint32_t* spec_state() { return (uint32_t*)(spec_rec + SL.state); }
 
-    // Publishes the remaining root children (orbit representatives at and
-    // after the cursor) for other workers to refine.  Called at level 1.
-    __device__ void spec_publish(const KParams& P, const LevelRec& r, uint32_t graph) {
+    // This graph will not publish (any more): it leaves the launch's count of
+    // graphs that may, which lets helpers leave once nothing is published.
+    __device__ void settle_pending(const KParams& P) {
         if (spec_pending) {
             spec_pending = false;
             if (lane_id() == 0) atomicSub(P.pending, 1u);
         }
+    }
+    // Publishes the remaining root children (orbit representatives at and
+    // after the cursor) for other workers to refine.  Called at level 1.  The
+    // graph leaves the pending count only once its record is on the board.
+    __device__ void spec_publish(const KParams& P, const LevelRec& r, uint32_t graph) {
         // count candidates
         uint32_t cnt = 0;
         for (uint32_t i = r.cursor + (uint32_t)lane_id(); i < r.tc_len; i += kWave)
             cnt += arena_rep[r.arena_off + i] == arena_mem[r.arena_off + i] ? 1u : 0u;
         cnt = wave_sum(cnt);
-        if (cnt < 2) return;
+        if (cnt < 2) {
+            settle_pending(P);
+            return;
+        }
         // take a free record
         int32_t slot = -1;
         if (lane_id() == 0) {
@@ -1651,7 +1659,10 @@ struct Worker {
             }
         }
         slot = __shfl(slot, 0, kWave);
-        if (slot < 0) return;
+        if (slot < 0) {
+            settle_pending(P);
+            return;
+        }
         spec_slot = slot;
         spec_rec = P.spec + (uint64_t)slot * P.spec_stride;
         SL = make_spec_layout(P.maxn, P.spec_slots);
@@ -1695,7 +1706,11 @@ struct Worker {
             agent_store(&hd->epoch, epoch);
             __threadfence();
             atomicOr(&P.board_mask[slot >> 6], (unsigned long long)(1ull << (slot & 63)));
+            // the record is on the board before the graph leaves the pending
+            // count: a helper that reads the count at 0 then finds the record
+            __threadfence();
         }
+        settle_pending(P);
         group_sync();
     }
 
@@ -1773,10 +1788,7 @@ struct Worker {
 
     // Ends the publication: withdraw unclaimed items, wait for running ones.
     __device__ void spec_finish(const KParams& P) {
-        if (spec_pending) {
-            spec_pending = false;
-            if (lane_id() == 0) atomicSub(P.pending, 1u);
-        }
+        settle_pending(P);
         if (spec_slot < 0) return;
         uint32_t* state = spec_state();
         const uint32_t E = spec_epoch << 2;
@@ -1956,10 +1968,18 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
                               : 0ull;
     const uint64_t nz = ballot(word != 0ull);
     if (!nz) {
-        uint32_t pend = 0;
-        if (l == 0) pend = agent_peek(P.pending);
+        uint32_t pend = 0, again = 0;
+        if (l == 0) {
+            pend = __hip_atomic_load(P.pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            // a graph publishes before it leaves the count: with the count
+            // at 0, a second look at the board sees every record published
+            if (pend == 0)
+                for (uint32_t w = 0; w < nwr && !again; ++w)
+                    again = __hip_atomic_load(&P.board_mask[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+        }
         pend = __shfl(pend, 0, kWave);
-        return pend == 0 ? -1 : 0;
+        again = __shfl(again, 0, kWave);
+        return pend == 0 && !again ? -1 : 0;
     }
     const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
     // spread helpers: start at a word and bit that depend on the worker

@@ -29,16 +29,19 @@ def test_gpu_large_er_matches_host_engine(n, seed):
     assert (r.nodes, r.automorphisms) == (c.nodes, c.automorphisms)
 
 
-def test_gpu_large_regular_invariance():
-    # a 3-regular graph with n = 4096 (1.6e7 splitter steps; beyond the host
-    # engine in test time): the canonical form and the search (node count) do
-    # not depend on the vertex names, and the labeling is a permutation
-    g = gcl.random_regular(4096, 3, 4)
-    _log("large rr3 n=4096")
+@pytest.mark.parametrize("n", [4096, 10000])
+def test_gpu_large_regular_invariance(n):
+    # 3-regular graphs (n + 1 search nodes, n^2 splitter steps: beyond the
+    # host engine in test time), in an LDS size class and in the global one:
+    # the canonical form and the search (node count) do not depend on the
+    # vertex names, and the labeling is a permutation
+    g = gcl.random_regular(n, 3, 4)
+    _log(f"large rr3 n={n}")
     r0 = gcl.canon_label(g, engine="gpu")
     assert np.array_equal(np.sort(r0.label), np.arange(g.n))
     assert r0.nodes == g.n + 1
     h = gcl.permute(g, 77)
+    _log(f"large rr3 n={n} relabeled")
     r1 = gcl.canon_label(h, engine="gpu")
     assert np.array_equal(r1.keys, r0.keys)
     assert r1.nodes == r0.nodes
