@@ -40,7 +40,7 @@ def log(msg):
 
 
 def kernel_name(max_n):
-    return f"search_kernel_lds<{max_n}>" if max_n in (64, 128, 256, 512, 1024, 2048, 4096) else "search_kernel_global"
+    return f"search_kernel_lds<{max_n}>" if max_n in (64, 128, 256, 512, 1024, 2048, 4096, 8192) else "search_kernel_global"
 
 
 def traffic_from_profile(kernel, graphs):

@@ -38,8 +38,8 @@ struct DevBuf {
     }
 };
 
-constexpr int kClasses = 7;
-constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048, 4096};
+constexpr int kClasses = 8;
+constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
 
 template <int MAXN>
 hipError_t launch_lds(const KParams& P, int grid, hipStream_t s) {
@@ -56,6 +56,7 @@ hipError_t launch_class(int cls, const KParams& P, int grid, hipStream_t s) {
         case 4: return launch_lds<1024>(P, grid, s);
         case 5: return launch_lds<2048>(P, grid, s);
         case 6: return launch_lds<4096>(P, grid, s);
+        case 7: return launch_lds<8192>(P, grid, s);
         default:
             hipLaunchKernelGGL(search_kernel_global, dim3(grid), dim3(kWave), 0, s, P);
             return hipGetLastError();
@@ -71,6 +72,7 @@ const void* class_kernel(int cls) {
         case 4: return (const void*)search_kernel_lds<1024>;
         case 5: return (const void*)search_kernel_lds<2048>;
         case 6: return (const void*)search_kernel_lds<4096>;
+        case 7: return (const void*)search_kernel_lds<8192>;
         default: return (const void*)search_kernel_global;
     }
 }
@@ -495,6 +497,7 @@ hipError_t launch_serve(int cls, const KParams& P, int grid, hipStream_t s) {
         case 4: return launch_serve_lds<1024>(P, grid, s);
         case 5: return launch_serve_lds<2048>(P, grid, s);
         case 6: return launch_serve_lds<4096>(P, grid, s);
+        case 7: return launch_serve_lds<8192>(P, grid, s);
         default:
             hipLaunchKernelGGL(serve_kernel_global, dim3(grid), dim3(kWave), 0, s, P);
             return hipGetLastError();
@@ -510,6 +513,7 @@ const void* serve_kernel(int cls) {
         case 4: return (const void*)serve_kernel_lds<1024>;
         case 5: return (const void*)serve_kernel_lds<2048>;
         case 6: return (const void*)serve_kernel_lds<4096>;
+        case 7: return (const void*)serve_kernel_lds<8192>;
         default: return (const void*)serve_kernel_global;
     }
 }
@@ -564,7 +568,7 @@ struct ServeEngine {
         GCL_HIP(hipGetDeviceProperties(&prop, 0));
         const uint32_t ring_slots = std::max(1u, std::min(cfg.max_aut, cfg.ring_cap));
         // serving kernels: the size class of min(max_n, 1024), and for larger
-        // graphs the class of max_n (2048, 4096 or the global one)
+        // graphs the class of max_n (2048, 4096, 8192 or the global one)
         auto class_for = [](uint32_t n) {
             for (int k = 0; k < kClasses; ++k)
                 if (n <= kClassN[k]) return k;

@@ -19,10 +19,21 @@ def _log(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("n,seed", [(20000, 3), (50000, 5)])
+@pytest.mark.parametrize("n,seed", [(6000, 7), (20000, 3), (50000, 5)])
 def test_gpu_large_er_matches_host_engine(n, seed):
     g = gcl.permute(gcl.erdos_renyi_mean_degree(n, 8.0, seed), seed + 100)
     _log(f"large er8 n={n} m={g.m}")
+    r = gcl.canon_label(g, engine="gpu")
+    c = gcl.canon_label(g, engine="cpu")
+    assert np.array_equal(r.keys, c.keys)
+    assert (r.nodes, r.automorphisms) == (c.nodes, c.automorphisms)
+
+
+def test_gpu_large_torus_matches_host_engine():
+    # 80 x 80 torus (n = 6400, the 8192 LDS class): a vertex-transitive graph
+    # whose search prunes by automorphisms
+    g = gcl.permute(gcl.torus(80, 80), 31)
+    _log(f"large torus80x80 n={g.n}")
     r = gcl.canon_label(g, engine="gpu")
     c = gcl.canon_label(g, engine="cpu")
     assert np.array_equal(r.keys, c.keys)
