@@ -457,6 +457,7 @@ struct Worker {
     uint64_t nodes, steps, csr_bytes, budget;
     int32_t status;
     // speculation (owner side)
+    bool root_changed;       // the partition held at level 1 is not the root's (own helping)
     int32_t spec_slot;       // board slot published by this worker, or -1
     bool spec_pending;       // counted in *pending until it publishes or ends
     uint32_t spec_epoch, spec_cursor, spec_count;
@@ -1201,6 +1202,7 @@ struct Worker {
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = src[w];
         H.sync();
         rebuild_from_lasts();
+        if (lev == 1) root_changed = false;
     }
 
     // --------------------------------------------------------- search ops
@@ -1715,9 +1717,12 @@ struct Worker {
     }
 
     // Root child v: adopt a helper's refinement, or claim and refine it here.
-    // Returns true when v was handled (partition at level 2, code set).
-    __device__ bool spec_take(const KParams& P, uint32_t v) {
-        if (spec_slot < 0) return false;
+    // Returns kTaken when v was adopted (partition at level 2, code set),
+    // kRejected when its code alone decides it (see below), kNotTaken when
+    // it must be refined here.
+    static constexpr int kNotTaken = 0, kTaken = 1, kRejected = 2;
+    __device__ int spec_take(const KParams& P, uint32_t v) {
+        if (spec_slot < 0) return kNotTaken;
         const ix_t* vert = (const ix_t*)(spec_rec + SL.vert);
         uint32_t* state = spec_state();
         const uint32_t E = spec_epoch << 2;
@@ -1726,37 +1731,56 @@ struct Worker {
             if (lane_id() == 0) agent_cas(&state[spec_cursor], E | 0u, E | 3u);
             ++spec_cursor;
         }
-        if (spec_cursor >= spec_count || (uint32_t)vert[spec_cursor] != v) return false;
+        if (spec_cursor >= spec_count || (uint32_t)vert[spec_cursor] != v) return kNotTaken;
         const uint32_t j = spec_cursor++;
         uint32_t won = 0;
         if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 3u) ? 1u : 0u;
         won = __shfl(won, 0, kWave);
-        if (won) return false;  // withdrawn: refine it here
+        if (won) return kNotTaken;  // withdrawn: refine it here
         // a helper has it: refine later children meanwhile, then copy its partition
         while (agent_peek(&state[j]) != (E | 2u)) {
             if (wall_clock64() > deadline) {
                 status = GCL_E_INTERNAL;  // a helper did not finish in time
-                return false;
+                return kNotTaken;
             }
             if (!spec_help_own(P, E)) __builtin_amdgcn_s_sleep(2);
         }
         agent_acquire();
+        const uint32_t hc = uni(((const uint32_t*)(spec_rec + SL.code))[j]);
+        // A child whose code is below the best path's at this level and
+        // differs from the first path's is classified kBad before its
+        // partition is looked at (classify_node, reference
+        // src/sequential.cpp:396-624), and the backtrack that follows puts
+        // back the root partition this worker still holds.  In the root
+        // state (level 1, eqlevcanon 1, comp_canon 0: every root child after
+        // the first path returns to it) such a child costs its counters only,
+        // unless this worker refined a child of its own record meanwhile (the
+        // partition it holds is then that child's).
+        if (hc != kFailedCode && !root_changed && level == 1 && seq_size == 0 && eqlevfirst <= 1 &&
+            eqlevcanon == 1 && comp_canon == 0) {
+            const bool eq_first = eqlevfirst == 1 && (int)hc == uni(firstcode[1]);
+            if (!eq_first && (int)hc < uni(canon_code[1])) {
+                steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
+                csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
+                tc_level_reached = 1;  // what backtrack(1) leaves (the rest is unchanged)
+                return kRejected;
+            }
+        }
+        if (hc == kFailedCode) {
+            status = GCL_E_INTERNAL;  // the helper's refinement failed
+            return kNotTaken;
+        }
         const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
         const uint64_t* il = (const uint64_t*)(spec_rec + SL.lasts) + (uint64_t)j * (P_nw1());
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = il[w];
-        const uint32_t hc = ((const uint32_t*)(spec_rec + SL.code))[j];
-        if (hc == kFailedCode) {
-            status = GCL_E_INTERNAL;  // the helper's refinement failed
-            return false;
-        }
         code = (int)hc;
         steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
         csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
         H.sync();
         positions_from_elem();
         rebuild_from_lasts();
-        return true;
+        return kTaken;
     }
     // While a helper refines the child this worker waits for, claim a later
     // unclaimed child of its own record and refine it as a helper would.
@@ -1775,6 +1799,7 @@ struct Worker {
         if (idx == 0xFFFFFFFFu) return false;
         const uint64_t st0 = steps, by0 = csr_bytes, bud0 = budget;
         const int32_t s0 = status;
+        root_changed = true;  // help_item works in this worker's partition
         help_item(P, spec_rec, idx, E);
         steps = st0;
         csr_bytes = by0;
@@ -1844,7 +1869,13 @@ struct Worker {
             if (level == 1 && spec_pending) spec_publish(P, r, graph);
             const uint32_t v = arena_mem[r.arena_off + cur];
             lv[li].cursor = cur + 1;
-            if (level == 1 && spec_take(P, v)) {
+            const int taken = level == 1 ? spec_take(P, v) : kNotTaken;
+            if (taken == kRejected) {
+                // classified and backtracked already (spec_take)
+                ++nodes;
+                return true;
+            }
+            if (taken == kTaken) {
                 ++nodes;
                 seq[seq_size] = (ix_t)v;
                 ++seq_size;
@@ -1877,6 +1908,7 @@ struct Worker {
         first_discovered = false;
         budget = step_budget;
         spec_slot = -1;
+        root_changed = false;
         spec_maxn = P.maxn;
         // counted in *pending by the host before the launch
         spec_pending = P.spec_min_n > 0 && n >= P.spec_min_n;
