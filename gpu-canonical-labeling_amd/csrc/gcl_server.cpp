@@ -9,6 +9,7 @@
 #include <deque>
 #include <mutex>
 #include <new>
+#include <stdexcept>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -137,24 +138,38 @@ struct Server {
         }
         if (fp > cfg.queue_bytes) return fail(GCL_E_CAPACITY, "request footprint exceeds queue_bytes");
         {
+            // the footprint is reserved with the check, so concurrent
+            // submitters cannot together pass queue_bytes
             std::lock_guard<std::mutex> lk(mu);
             if (fatal != GCL_OK) return fail(fatal, "server stopped after a device error");
             if (queued_bytes + fp > cfg.queue_bytes) {
                 ++st.refused_busy;
                 return fail(GCL_E_BUSY, "server queue full");
             }
+            queued_bytes += fp;
         }
         Request* q = new (std::nothrow) Request();
-        if (!q) return fail(GCL_E_NOMEM, "out of host memory");
-        q->ngraphs = ng;
-        for (int i = 0; i < ng; ++i) q->g[i] = copy_graph(*gs[i]);
+        bool copied = q != nullptr;
+        if (q) {
+            try {
+                q->ngraphs = ng;
+                for (int i = 0; i < ng; ++i) q->g[i] = copy_graph(*gs[i]);
+            } catch (const std::bad_alloc&) {
+                copied = false;
+            }
+        }
+        if (!copied) {
+            delete q;
+            std::lock_guard<std::mutex> lk(mu);
+            queued_bytes -= fp;
+            return fail(GCL_E_NOMEM, "out of host memory");
+        }
         q->out_label = out_label;
         q->t0 = t0;
         q->footprint = fp;
         {
             std::lock_guard<std::mutex> lk(mu);
             q->ticket = next_ticket++;
-            queued_bytes += fp;
             ++st.submitted;
             pending.push_back(q);
             *ticket = q->ticket;
@@ -437,6 +452,21 @@ struct Server {
 
 using gcl::fail;
 
+namespace {
+// Runs f, turning a C++ exception (thread creation, container growth) into a
+// status: nothing is thrown across the C ABI.
+template <class F>
+gcl_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(GCL_E_NOMEM, "out of host memory");
+    } catch (const std::exception& e) {
+        return fail(GCL_E_INTERNAL, "server: %s", e.what());
+    }
+}
+}  // namespace
+
 extern "C" {
 
 void gcl_server_config_default(gcl_server_config* c) {
@@ -454,7 +484,7 @@ gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out) {
     if (gcl::device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
     gcl::Server* s = new (std::nothrow) gcl::Server();
     if (!s) return fail(GCL_E_NOMEM, "out of host memory");
-    gcl_status r = s->create(*cfg);
+    gcl_status r = guarded([&] { return s->create(*cfg); });
     if (r != GCL_OK) {
         delete s;
         return r;
@@ -488,7 +518,7 @@ gcl_status gcl_server_replay(gcl_server* s, const gcl_batch* graphs, const int32
     if (pair_with)
         for (int32_t i = 0; i < count; ++i)
             if (pair_with[i] >= graphs->num_graphs) return fail(GCL_E_INVALID, "pair index out of range");
-    return reinterpret_cast<gcl::Server*>(s)->replay(graphs, pair_with, arrival_s, count, out);
+    return guarded([&] { return reinterpret_cast<gcl::Server*>(s)->replay(graphs, pair_with, arrival_s, count, out); });
 }
 
 void gcl_server_destroy(gcl_server* s) { delete reinterpret_cast<gcl::Server*>(s); }
