@@ -142,6 +142,11 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
     return GCL_OK;
 }
 
+gcl_status gcl_set_side_files(const char* prefix) {
+    gcl::set_side_file_prefix(prefix);
+    return GCL_OK;
+}
+
 gcl_status gcl_nauty_parallel(const gcl_csr* g, int32_t max_aut, uint8_t* out_canon, uint32_t* node_count) {
     if (max_aut < 1) return fail(GCL_E_INVALID, "max_aut must be at least 1");
     gcl_stats st;

@@ -1,9 +1,27 @@
 // gcl_device.h -- host-side interface of the device engine (internal).
 #pragma once
 
+#include <string>
+
 #include "gcl_internal.h"
 
 namespace gcl {
+
+// Per-worker diagnostics of one launch, written when the launch is given a
+// record array (side files): where the worker ran and how its time went.
+// Times are device wall-clock ticks.
+struct DiagRecord {
+    uint32_t hw_id;    // __smid(): XCC in bits 9:6, shader engine in 5:4, CU in 3:0 (gfx950)
+    uint32_t graphs;   // graphs this worker searched
+    uint64_t nodes;    // their search-tree nodes
+    uint64_t steps;    // their splitter steps (with root children refined for them)
+    uint64_t helped;   // root children this worker refined for other workers
+    uint64_t t_total;  // whole loop
+    uint64_t t_search; // searching its own graphs (t_wait included)
+    uint64_t t_wait;   // within t_search: waiting for root children from helpers
+    uint64_t t_help;   // refining root children for other workers
+    uint64_t t_idle;   // backing off with nothing to do
+};
 
 // Per-graph result of a device launch.
 struct DeviceGraphResult {
@@ -86,6 +104,10 @@ gcl_status serve_check(ServeEngine* s);  // GCL_E_DEVICE once a serving kernel h
 int32_t device_count();
 uint64_t device_wall_ticks_per_ms();
 uint64_t step_budget();
+// Side files written after each search launch (empty: none); see
+// gcl_set_side_files in gcl.h.
+void set_side_file_prefix(const char* prefix);
+std::string side_file_prefix();
 // Smallest graph whose root children are published to helpers
 // (GCL_SPEC_MIN_N overrides; 0 disables).
 uint32_t spec_min_n(uint32_t dflt = 128);

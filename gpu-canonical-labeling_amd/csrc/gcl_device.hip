@@ -4,9 +4,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <numeric>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "gcl_device.h"
@@ -89,6 +94,21 @@ __global__ void reset_counters(unsigned int* ctr, const unsigned int* init, int 
 // Per-graph bound on splitter steps plus search iterations; a graph that
 // exhausts it reports GCL_E_INTERNAL instead of running on.  GCL_STEP_BUDGET
 // overrides the default.
+namespace {
+std::mutex g_side_mu;
+std::string g_side_prefix;
+}  // namespace
+
+void set_side_file_prefix(const char* prefix) {
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    g_side_prefix = prefix ? prefix : "";
+}
+
+std::string side_file_prefix() {
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    return g_side_prefix;
+}
+
 uint64_t step_budget() {
     const char* e = getenv("GCL_STEP_BUDGET");
     if (e && *e) return strtoull(e, nullptr, 10);
@@ -136,7 +156,7 @@ struct DeviceBatch {
         int64_t grid = 0;
         size_t item_off = 0;
         KParams P;
-        DevBuf scratch, spec, board;
+        DevBuf scratch, spec, board, diag;
         uint64_t spec_work = 0;  // root children of the class's large graphs
         uint32_t spec_big = 0;   // graphs of the class that may publish
         hipStream_t stream = nullptr;
@@ -155,6 +175,7 @@ struct DeviceBatch {
     hipGraphExec_t exec = nullptr;
     int32_t workers = 0;
     bool timed_run = false;  // last run carried per-class events
+    std::string side;        // side-file prefix taken at creation ("" = none)
 
     ~DeviceBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -172,6 +193,7 @@ struct DeviceBatch {
     }
 
     gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap) {
+        side = side_file_prefix();
         if (max_aut_ == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
         G = b.num_graphs;
         if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
@@ -303,6 +325,10 @@ struct DeviceBatch {
                                   hipMemcpyHostToDevice));
             }
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
+            if (!side.empty()) {
+                GCL_HIP(hipMalloc(&C.diag.p, sizeof(DiagRecord) * (size_t)C.grid));
+                GCL_HIP(hipMemset(C.diag.p, 0, sizeof(DiagRecord) * (size_t)C.grid));
+            }
             GCL_HIP(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
             GCL_HIP(hipEventCreateWithFlags(&C.done, hipEventDisableTiming));
             GCL_HIP(hipEventCreate(&C.t0));
@@ -338,6 +364,7 @@ struct DeviceBatch {
             P.slot_free = spec ? (unsigned long long*)C.board.p : nullptr;
             P.board_mask = spec ? (unsigned long long*)C.board.p + (spec_records + 63) / 64 : nullptr;
             P.pending = (unsigned int*)ctr.p + 32 + cls;
+            P.diag = side.empty() ? nullptr : (DiagRecord*)C.diag.p;
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));
@@ -404,6 +431,45 @@ struct DeviceBatch {
         float t = 0.f;
         GCL_HIP(hipEventElapsedTime(&t, ev0, ev1));
         if (ms) *ms = t;
+        if (!side.empty()) return write_side_files();
+        return GCL_OK;
+    }
+
+    // Side files of the last run (counterpart of the reference's per-block
+    // CSVs, src/nauty_parallel.cu:882-981): <prefix>_counters.csv, one row
+    // per worker of every size-class kernel, and <prefix>_nodes_per_cu.csv.
+    gcl_status write_side_files() {
+        std::vector<std::pair<int, DiagRecord>> rows;
+        for (auto& c : classes) {
+            std::vector<DiagRecord> r((size_t)c.grid);
+            GCL_HIP(hipMemcpy(r.data(), c.diag.p, sizeof(DiagRecord) * r.size(), hipMemcpyDeviceToHost));
+            for (auto& x : r) rows.emplace_back((int)c.P.maxn, x);
+        }
+        const std::string fc = side + "_counters.csv", fn = side + "_nodes_per_cu.csv";
+        FILE* f = fopen(fc.c_str(), "w");
+        if (!f) return fail(GCL_E_IO, "cannot write %s", fc.c_str());
+        fprintf(f, "WORKER,CLASS_MAX_N,XCC,SE,CU,GRAPHS,NODES,SPLITTER_STEPS,HELPED_CHILDREN,"
+                   "SEARCH_pct,WAIT_CHILDREN_pct,HELP_pct,IDLE_pct,TOTAL_TICKS\n");
+        std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint64_t> per_cu;
+        for (size_t i = 0; i < rows.size(); ++i) {
+            const DiagRecord& d = rows[i].second;
+            const uint32_t xcc = d.hw_id >> 6, se = (d.hw_id >> 4) & 3u, cu = d.hw_id & 15u;
+            const double tot = d.t_total ? (double)d.t_total : 1.0;
+            const uint64_t own = d.t_search > d.t_wait ? d.t_search - d.t_wait : 0;
+            fprintf(f, "%zu,%d,%u,%u,%u,%u,%llu,%llu,%llu,%.2f,%.2f,%.2f,%.2f,%llu\n", i, rows[i].first, xcc, se, cu,
+                    d.graphs, (unsigned long long)d.nodes, (unsigned long long)d.steps,
+                    (unsigned long long)d.helped, 100.0 * (double)own / tot, 100.0 * (double)d.t_wait / tot,
+                    100.0 * (double)d.t_help / tot, 100.0 * (double)d.t_idle / tot, (unsigned long long)d.t_total);
+            per_cu[std::make_tuple(xcc, se, cu)] += d.nodes;
+        }
+        fclose(f);
+        f = fopen(fn.c_str(), "w");
+        if (!f) return fail(GCL_E_IO, "cannot write %s", fn.c_str());
+        fprintf(f, "xcc,se,cu,nodes\n");
+        for (auto& kv : per_cu)
+            fprintf(f, "%u,%u,%u,%llu\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
+                    (unsigned long long)kv.second);
+        fclose(f);
         return GCL_OK;
     }
 

@@ -189,6 +189,7 @@ struct KParams {
     unsigned long long* board_mask;  // bit r set while record r is published
     unsigned int* pending;      // running graphs that may still publish
     uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
+    DiagRecord* diag;           // per-worker records (grid entries), or NULL
     ServeParams sv;             // serving mode when sv.tail != nullptr
 };
 
@@ -458,6 +459,9 @@ struct Worker {
     int32_t status;
     // speculation (owner side)
     bool root_changed;       // the partition held at level 1 is not the root's (own helping)
+    // diagnostics: totals of this worker over the launch (DiagRecord)
+    uint64_t dg_nodes, dg_steps, dg_helped, dg_search, dg_wait, dg_help, dg_idle;
+    uint32_t dg_graphs;
     int32_t spec_slot;       // board slot published by this worker, or -1
     bool spec_pending;       // counted in *pending until it publishes or ends
     uint32_t spec_epoch, spec_cursor, spec_count;
@@ -1738,6 +1742,7 @@ struct Worker {
         won = __shfl(won, 0, kWave);
         if (won) return kNotTaken;  // withdrawn: refine it here
         // a helper has it: refine later children meanwhile, then copy its partition
+        const uint64_t tw = wall_clock64();
         while (agent_peek(&state[j]) != (E | 2u)) {
             if (wall_clock64() > deadline) {
                 status = GCL_E_INTERNAL;  // a helper did not finish in time
@@ -1745,6 +1750,7 @@ struct Worker {
             }
             if (!spec_help_own(P, E)) __builtin_amdgcn_s_sleep(2);
         }
+        dg_wait += wall_clock64() - tw;
         agent_acquire();
         const uint32_t hc = uni(((const uint32_t*)(spec_rec + SL.code))[j]);
         // A child whose code is below the best path's at this level and
@@ -2054,6 +2060,18 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
     return 1;
 }
 
+// help_once, with its time and the children refined counted for diagnostics.
+template <class Hot>
+__device__ int help_counted(Worker<Hot>& W, const KParams& P) {
+    const uint64_t t0 = wall_clock64();
+    const int h = help_once(W, P);
+    if (h > 0) {
+        W.dg_help += wall_clock64() - t0;
+        ++W.dg_helped;
+    }
+    return h;
+}
+
 template <class Hot>
 __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, const Layout& L) {
     W.snap = (uint64_t*)(base + L.snap);
@@ -2080,6 +2098,26 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     const uint64_t now = wall_clock64();
     W.deadline = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
     W.steps = 0;
+    W.dg_nodes = W.dg_steps = W.dg_helped = W.dg_search = W.dg_wait = W.dg_help = W.dg_idle = 0;
+    W.dg_graphs = 0;
+}
+
+// Writes this worker's diagnostics record (lane 0), when the launch has them.
+template <class Hot>
+__device__ void write_diag(Worker<Hot>& W, const KParams& P, uint64_t t_start) {
+    if (P.diag == nullptr || lane_id() != 0) return;
+    DiagRecord r;
+    r.hw_id = __smid();
+    r.graphs = W.dg_graphs;
+    r.nodes = W.dg_nodes;
+    r.steps = W.dg_steps;
+    r.helped = W.dg_helped;
+    r.t_total = wall_clock64() - t_start;
+    r.t_search = W.dg_search;
+    r.t_wait = W.dg_wait;
+    r.t_help = W.dg_help;
+    r.t_idle = W.dg_idle;
+    P.diag[blockIdx.x] = r;
 }
 
 // Searches graph g (vertices vtx_off[g] .. vtx_off[g+1]-1) and writes its
@@ -2098,6 +2136,10 @@ __device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
     const uint64_t t0 = wall_clock64();
     W.run(P, (uint32_t)g, vbase);
     const uint64_t t1 = wall_clock64();
+    W.dg_search += t1 - t0;
+    W.dg_nodes += W.nodes;
+    W.dg_steps += W.steps;
+    ++W.dg_graphs;
     for (uint32_t i = (uint32_t)l; i < W.n; i += kWave) P.out_label[vbase + i] = (int32_t)W.best_label[i];
     if (l == 0) {
         DeviceGraphResult res;
@@ -2116,12 +2158,13 @@ __device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
 template <class Hot>
 __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
+    const uint64_t t_start = wall_clock64();
     bool graphs_left = true;
     uint32_t idle = 1;
     while (true) {
         // help-first: root children of a published large graph are on the
         // batch's critical path; a new graph is taken when there are none
-        if (P.spec_min_n && help_once(W, P) > 0) continue;
+        if (P.spec_min_n && help_counted(W, P) > 0) continue;
         if (graphs_left) {
             uint32_t item = 0;
             if (l == 0) item = atomicAdd(P.next_item, 1u);
@@ -2133,16 +2176,19 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             graphs_left = false;
         }
         if (P.spec_min_n == 0 || wall_clock64() > W.deadline) break;
-        const int h = help_once(W, P);
+        const int h = help_counted(W, P);
         if (h < 0) break;
         if (h == 0) {
             // idle: back off (up to ~100 us) to keep the polling traffic low
+            const uint64_t ti = wall_clock64();
             for (uint32_t i = 0; i < idle; ++i) __builtin_amdgcn_s_sleep(127);
+            W.dg_idle += wall_clock64() - ti;
             idle = idle < 32 ? idle * 2 : 32;
         } else {
             idle = 1;
         }
     }
+    write_diag(W, P, t_start);
 }
 
 // Poll of host-coherent memory.  Relaxed: an acquire at system scope would

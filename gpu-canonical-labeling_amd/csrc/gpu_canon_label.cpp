@@ -10,6 +10,8 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -72,13 +74,24 @@ struct Canon {
     bool ok = false;
 };
 
-Canon canonicalize(const gcl_graph* g, uint32_t max_aut, bool parallel) {
+// GCL_SIDE_FILES set (and not "0"): parallel runs write the side files of
+// gcl_set_side_files, named after the graph file's stem as the reference
+// names its per-run CSVs (main.cpp:127-131).
+bool side_files_wanted() {
+    const char* e = std::getenv("GCL_SIDE_FILES");
+    return e && *e && std::string(e) != "0";
+}
+
+Canon canonicalize(const gcl_graph* g, uint32_t max_aut, bool parallel, const std::string& file) {
     Canon c;
     c.n = g->csr.n;
     c.keys.assign((size_t)g->csr.nnz, 0);
     gcl_stats st;
+    const bool side = parallel && side_files_wanted();
+    if (side) gcl_set_side_files(std::filesystem::path(file).stem().string().c_str());
     gcl_status s = gcl_canon_label(&g->csr, max_aut, parallel ? GCL_ENGINE_GPU : GCL_ENGINE_CPU, nullptr,
                                    c.keys.data(), nullptr, &st);
+    if (side) gcl_set_side_files(nullptr);
     if (s != GCL_OK) {
         std::cerr << "[ERROR] " << gcl_last_error() << "\n";
         return c;
@@ -138,7 +151,7 @@ int main(int argc, char* argv[]) {
     }
     int rc = 0;
     if (graphs.size() == 1) {
-        Canon c = canonicalize(graphs[0], max_aut, use_parallel);
+        Canon c = canonicalize(graphs[0], max_aut, use_parallel, graph_files[0]);
         if (!c.ok) {
             std::cerr << "[ERROR] Canonical labeling failed.\n";
             rc = 1;
@@ -157,12 +170,12 @@ int main(int argc, char* argv[]) {
     } else {
         std::cout << "\nGraph 1: " << graph_files[0] << "\n";
         std::cout << "Computing canonical form...\n";
-        Canon c1 = canonicalize(graphs[0], max_aut, use_parallel);
+        Canon c1 = canonicalize(graphs[0], max_aut, use_parallel, graph_files[0]);
         Canon c2;
         if (c1.ok) {
             std::cout << "\nGraph 2: " << graph_files[1] << "\n";
             std::cout << "Computing canonical form...\n";
-            c2 = canonicalize(graphs[1], max_aut, use_parallel);
+            c2 = canonicalize(graphs[1], max_aut, use_parallel, graph_files[1]);
         }
         if (!c1.ok || !c2.ok) {
             std::cerr << "[ERROR] Canonical labeling failed.\n";

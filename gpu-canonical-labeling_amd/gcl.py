@@ -110,6 +110,7 @@ _SIGS = {
     "gcl_gen_permute": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "gcl_gen_edge_swap": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "gcl_device_count": (C.c_int32, []),
+    "gcl_set_side_files": (C.c_int, [C.c_char_p]),
     "gcl_device_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -488,6 +489,12 @@ def batch_canon(batch: Batch, max_aut=10000, engine="gpu", plain=False) -> Batch
                                      C.byref(st)))
     return BatchResult(labels[:int(batch.vtx_off[-1])], nodes[:batch.num_graphs], st.nodes, st.automorphisms,
                        st.splitter_steps, st.csr_bytes, st.kernel_ms, st.total_ms, st.workers)
+
+
+def set_side_files(prefix):
+    """Side files after each GPU search launch (gcl_set_side_files): <prefix>_counters.csv and
+    <prefix>_nodes_per_cu.csv; None or "" turns them off."""
+    _check(lib().gcl_set_side_files(prefix.encode() if prefix else None))
 
 
 def device_count():

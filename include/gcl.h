@@ -80,6 +80,23 @@ gcl_status gcl_nauty_parallel(const gcl_csr* g, int32_t max_aut,
                               uint8_t* out_canon, uint32_t* node_count);
 
 /*
+ * Side files, the counterpart of Nauty_Parallel's output_prefix (reference
+ * src/nauty_parallel.h:9-13) and of the per-block CSVs it writes
+ * (nauty_parallel.cu:882-981).  With a prefix set, every GPU search launch
+ * started afterwards (gcl_nauty_parallel, gcl_canon_label, the batch entry
+ * points) writes, after each run:
+ *   <prefix>_counters.csv      one row per worker of each size-class kernel:
+ *                              XCC/SE/CU, graphs, nodes, splitter steps, root
+ *                              children refined for others, and the share of
+ *                              its time spent searching, waiting for root
+ *                              children, helping others and idle
+ *   <prefix>_nodes_per_cu.csv  search-tree nodes per CU
+ * NULL or "" (the default) writes none.  The setting is process-wide; calls
+ * that run at the same time with one prefix overwrite each other's files.
+ */
+gcl_status gcl_set_side_files(const char* prefix);
+
+/*
  * Replaces Nauty_Sequential + best_leaf.hash_of_perm_graph (reference
  * src/sequential.h:14,30; read at main.cpp:133-142): the same search on the
  * host.  Unlike the reference it does not free the caller's graph.
