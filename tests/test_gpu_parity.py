@@ -106,6 +106,25 @@ def test_gpu_batch_matches_oracle_and_plain():
         assert np.array_equal(batch.labels_of(plain.labels, i), lab), i
 
 
+def test_gpu_batch_coloured_mix_matches_oracle():
+    # coloured graphs (an initial partition of several cells, vertices in
+    # (colour, id) order) next to uncoloured ones in one batch, some large
+    # enough to publish root children
+    rs = np.random.RandomState(11)
+    graphs = []
+    for i, g in enumerate(_mixed_graphs(24, 9) + [gcl.random_regular(600, 3, 4), gcl.torus(20, 30)]):
+        if i % 2 == 0:
+            g = gcl.Graph(g.n, g.row_ptr, g.col, rs.randint(0, 3, size=g.n).astype(np.int32))
+        graphs.append(g)
+    _log(f"coloured batch of {len(graphs)}")
+    res = gcl.batch_canon(gcl.Batch(graphs), engine="gpu")
+    batch = gcl.Batch(graphs)
+    for i, g in enumerate(graphs):
+        o = oracle.canon_graph(g, max_aut=64)
+        keys = gcl.canonical_keys(g, batch.labels_of(res.labels, i))
+        assert np.array_equal(keys, o["keys"]), (i, g.n, g.color is not None)
+
+
 def test_gpu_batch_result_same_alone_and_in_mix():
     graphs = _mixed_graphs(60, 11)
     batch = gcl.Batch(graphs)
