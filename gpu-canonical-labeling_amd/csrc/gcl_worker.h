@@ -788,6 +788,109 @@ struct Worker {
         return nfrag;
     }
 
+    // Split of a large touched cell [f, c] under a splitter of more than one
+    // vertex when few of its members are touched (at most a wave, under a
+    // quarter of the cell).  Fragments go in increasing count, so the
+    // untouched members (count 0) stay a block at the front and the touched
+    // ones form the tail [B, c], grouped by count: touched members outside
+    // the tail swap with untouched ones inside it (ranks from ballots), the
+    // tail is ordered by count, and the front block only has its cell end
+    // rewritten.  The fragments, their order and the cell arrays end as the
+    // whole-cell counting sort leaves them (the order inside a fragment is
+    // unobservable).  Returns false, having changed nothing but scratch in
+    // tmp[f, c], when the cell does not qualify.
+    __device__ bool sparse_split(Ctx& X, uint32_t f, uint32_t c, uint32_t L, uint32_t ntv) {
+        const int l = lane_id();
+        // touched members of this cell, listed in tmp[f, f + tc)
+        uint32_t tc = 0;
+        for (uint32_t base = 0; base < ntv; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            bool in = false;
+            uint32_t u = 0;
+            if (i < ntv) {
+                u = H.tv(i);
+                in = H.cend(H.pos(u)) == c;
+            }
+            const uint64_t m = ballot(in);
+            const uint32_t at = tc + lanes_below(m);
+            if (in && at < L) H.tmp(f + at) = (ix_t)u;
+            tc += (uint32_t)__popcll(m);
+        }
+        tc = uni(tc);
+        if (tc == 0 || tc > (uint32_t)kWave || 4 * tc >= L) return false;
+        H.sync();
+        const uint32_t B = c - tc + 1;  // first position of the touched block
+        const bool act = (uint32_t)l < tc;
+        uint32_t u = 0, q = 0, hv = 0;
+        bool out = false, hole = false;
+        if (act) {
+            u = H.tmp(f + (uint32_t)l);
+            q = H.pos(u);
+            out = q < B;
+            hv = H.elem(B + (uint32_t)l);
+            hole = H.cnt_ld(hv) == 0u;
+        }
+        const uint64_t om = ballot(out), hm = ballot(hole);
+        // outsider of rank r swaps with the hole of rank r (as many of each)
+        if (out) {
+            const uint32_t r = lanes_below(om);
+            H.tmp(f + tc + r) = (ix_t)q;
+            H.tmp(f + 2 * tc + r) = (ix_t)u;
+        }
+        H.sync();
+        if (hole) {
+            const uint32_t r = lanes_below(hm);
+            const uint32_t qo = H.tmp(f + tc + r), uo = H.tmp(f + 2 * tc + r);
+            H.elem(B + (uint32_t)l) = (ix_t)uo;
+            H.pos(uo) = (ix_t)(B + (uint32_t)l);
+            H.elem(qo) = (ix_t)hv;
+            H.pos(hv) = (ix_t)qo;
+        }
+        H.sync();
+        // order the tail by count: one group per count value present
+        uint32_t v = 0, kv = 0;
+        if (act) {
+            v = H.elem(B + (uint32_t)l);
+            kv = H.cnt_ld(v);
+        }
+        const uint32_t kmin = uni(wave_min(act ? kv : 0xFFFFFFFFu));
+        const uint32_t kmax = uni(wave_max(act ? kv : 0u));
+        uint32_t newpos = 0, gstart = 0, gend = 0, run = 0, ngroups = 0;
+        for (uint32_t val = kmin; val <= kmax; ++val) {
+            const uint64_t m = ballot(act && kv == val);
+            if (!m) continue;
+            const uint32_t g = (uint32_t)__popcll(m);
+            if (act && kv == val) {
+                newpos = B + run + lanes_below(m);
+                gstart = B + run;
+                gend = B + run + g - 1;
+            }
+            run += g;
+            ++ngroups;
+        }
+        H.sync();
+        if (act) {
+            H.elem(newpos) = (ix_t)v;
+            H.pos(v) = (ix_t)newpos;
+            H.cend(newpos) = (ix_t)gend;
+            if (newpos == gstart) {
+                H.cfirst(gend) = (ix_t)gstart;
+                if (gend != c) set_bit_atomic(H.lasts(), gend);
+            }
+        }
+        // the untouched block [f, B - 1]
+        for (uint32_t pp = f + (uint32_t)l; pp < B; pp += kWave) H.cend(pp) = (ix_t)(B - 1);
+        if (l == 0) {
+            H.cfirst(B - 1) = (ix_t)f;
+            set_bit_atomic(H.lasts(), B - 1);
+            set_bit_atomic(H.split(), f);
+            set_bit_atomic(H.oldend(), c);
+        }
+        X.ncells += ngroups;
+        H.sync();
+        return true;
+    }
+
     // General refinement step: splitter cell [p, e] of any size (reference
     // src/partition_refinement.cpp:197-473): counts with LDS atomics; touched
     // cells regrouped by key (count for |W| > 1; neighbours first for
@@ -901,6 +1004,7 @@ struct Worker {
             const uint32_t c = uni((uint32_t)H.tcl(j));
             const uint32_t f = uni((uint32_t)H.cfirst(c));
             const uint32_t L = c - f + 1;
+            if (s > 1 && sparse_split(X, f, c, L, ntv)) continue;
             uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
