@@ -891,6 +891,93 @@ struct Worker {
         return true;
     }
 
+    // Hash chain and active-set rules of a step with a splitter of more than
+    // one vertex (reference src/partition_refinement.cpp:253-468), with the
+    // pre-step non-singleton cells listed in sb[0, nns) in position order.
+    // The chain's values are, in position order, one per cell that did not
+    // split (its common count plus its first position) and one per fragment
+    // of a cell that did (the fragment's count plus its first position).
+    // Each lane takes one cell: it walks its own fragments, writes its values
+    // at their place in the chain (hits, all zero here, as 16-bit entries)
+    // and applies the cell's active-set rules, which touch only the cell's
+    // positions; the chain itself then runs over the gathered values.  The
+    // hint ends as the last singleton fragment after the first, in position
+    // order.  LDS classes only (values below 2^16).
+    __device__ void hash_pass_parallel(Ctx& X, uint32_t nns, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        uint32_t nval = 0;
+        int32_t dact = 0;
+        for (uint32_t base = 0; base < nns; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            const bool valid = i < nns;
+            uint32_t f = 0, nv = 0;
+            bool sp = false;
+            if (valid) {
+                f = H.sb(i);
+                sp = test_bit(H.split(), f);
+                nv = 1;
+                if (sp) {
+                    nv = 0;
+                    for (uint32_t q = f;;) {
+                        const uint32_t last = H.cend(q);
+                        ++nv;
+                        if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                        q = last + 1;
+                    }
+                }
+            }
+            const uint32_t inc = wave_scan(nv);
+            uint32_t k = nval + inc - nv;
+            int32_t cand = -1;
+            if (valid && !sp) H.hits_st(k, H.cnt_ld(H.elem(f)) + f);
+            if (sp) {
+                const bool was_active = test_bit(H.active(), f);
+                uint32_t max_size = 0, max_first = f;
+                for (uint32_t q = f;;) {
+                    const uint32_t last = H.cend(q);
+                    const uint32_t size = last - q + 1;
+                    H.hits_st(k++, H.cnt_ld(H.elem(q)) + q);
+                    if (size > max_size) {
+                        max_size = size;
+                        max_first = q;
+                    }
+                    if (q != f) {
+                        const unsigned long long bit = 1ull << (q & 63);
+                        if (!(atomicOr((unsigned long long*)&H.active()[q >> 6], bit) & bit)) ++dact;
+                        if (last == q) cand = (int32_t)q;
+                    }
+                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                    q = last + 1;
+                }
+                if (!was_active) {
+                    const unsigned long long bf = 1ull << (f & 63), bm = 1ull << (max_first & 63);
+                    if (!(atomicOr((unsigned long long*)&H.active()[f >> 6], bf) & bf)) ++dact;
+                    if (atomicAnd((unsigned long long*)&H.active()[max_first >> 6], ~bm) & bm) --dact;
+                }
+            }
+            // the last candidate in position order wins
+            const uint64_t cm = ballot(cand >= 0);
+            if (cm) hint = (int)readlane((uint32_t)cand, 63 - __builtin_clzll(cm));
+            nval += readlane(inc, kWave - 1);
+        }
+        X.nactive += (uint32_t)uni((int32_t)wave_sum((uint32_t)dact));
+        H.sync();
+        for (uint32_t base = 0; base < nval; base += kWave) {
+            const uint32_t x = base + (uint32_t)l < nval ? H.hits_ld(base + (uint32_t)l) : 0u;
+            const uint32_t here = min((uint32_t)kWave, nval - base);
+            uint32_t t = 0;
+            for (; t + 4 <= here; t += 4) {
+                longcode = GCL_MASH(longcode, readlane(x, (int)t));
+                longcode = GCL_MASH(longcode, readlane(x, (int)t + 1));
+                longcode = GCL_MASH(longcode, readlane(x, (int)t + 2));
+                longcode = GCL_MASH(longcode, readlane(x, (int)t + 3));
+            }
+            for (; t < here; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
+        }
+        H.sync();
+        for (uint32_t b = (uint32_t)l; b < nval; b += kWave) H.hits_st(b, 0);
+    }
+
     // General refinement step: splitter cell [p, e] of any size (reference
     // src/partition_refinement.cpp:197-473): counts with LDS atomics; touched
     // cells regrouped by key (count for |W| > 1; neighbours first for
@@ -1093,6 +1180,9 @@ struct Worker {
         }
         H.sync();
         // hash chain / active rules in position order (reference 253-468)
+        if (Hot::kLds && s > 1) {
+            hash_pass_parallel(X, nns, longcode, hint);
+        } else
         for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             // cells that did not split hash their common count (0 when
