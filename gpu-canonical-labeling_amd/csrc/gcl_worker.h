@@ -900,9 +900,10 @@ struct Worker {
     // Each lane takes one cell: it walks its own fragments, writes its values
     // at their place in the chain (hits, all zero here, as 16-bit entries)
     // and applies the cell's active-set rules, which touch only the cell's
-    // positions; the chain itself then runs over the gathered values.  The
-    // hint ends as the last singleton fragment after the first, in position
-    // order.  LDS classes only (values below 2^16).
+    // positions, and updates the fragments' non-singleton marks and clears
+    // the cell's split marks; the chain itself then runs over the gathered
+    // values.  The hint ends as the last singleton fragment after the first,
+    // in position order.  LDS classes only (values below 2^16).
     __device__ void hash_pass_parallel(Ctx& X, uint32_t nns, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         uint32_t nval = 0;
@@ -946,9 +947,17 @@ struct Worker {
                         if (!(atomicOr((unsigned long long*)&H.active()[q >> 6], bit) & bit)) ++dact;
                         if (last == q) cand = (int32_t)q;
                     }
-                    if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                    // the fragment's non-singleton mark
+                    if (last > q) set_bit_atomic(H.nsmask(), q);
+                    else clear_bit_atomic(H.nsmask(), q);
+                    if (test_bit(H.oldend(), last)) {
+                        clear_bit_atomic(H.oldend(), last);
+                        break;
+                    }
+                    if (last + 1 >= X.n) break;
                     q = last + 1;
                 }
+                clear_bit_atomic(H.split(), f);
                 if (!was_active) {
                     const unsigned long long bf = 1ull << (f & 63), bm = 1ull << (max_first & 63);
                     if (!(atomicOr((unsigned long long*)&H.active()[f >> 6], bf) & bf)) ++dact;
@@ -1255,7 +1264,9 @@ struct Worker {
         }
         H.sync();
         // cell masks of the new fragments; clear the split marks.  The
-        // marks are keyed by old first positions: walk the ns list again.
+        // marks are keyed by old first positions: walk the ns list again
+        // (done by the parallel hash pass when it ran)
+        if (!(Hot::kLds && s > 1))
         for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             if (i < nns) {
