@@ -271,7 +271,7 @@ def main():
             out["served"] = served(args)
             out["served_vs_plain"] = round(out["served"]["value"] / plain_rate, 2)
             out["p99_latency_ms"] = out["served"]["latency_ms"]["p99"]
-        out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 256)
+        out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 128)
     print(json.dumps(out), flush=True)
 
 
