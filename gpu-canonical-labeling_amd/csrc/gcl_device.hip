@@ -156,7 +156,7 @@ struct DeviceBatch {
         int64_t grid = 0;
         size_t item_off = 0;
         KParams P;
-        DevBuf scratch, spec, board, diag;
+        DevBuf scratch, spec, board, diag, autpool;
         uint64_t spec_work = 0;  // root children of the class's large graphs
         uint32_t spec_big = 0;   // graphs of the class that may publish
         hipStream_t stream = nullptr;
@@ -288,16 +288,23 @@ struct DeviceBatch {
             const bool hot_global = cls == kClasses;
             // per-level snapshots of the cell-boundary bitmap (depth <= n)
             const uint32_t levels = std::min<uint32_t>(maxn + 2, 4096u);
-            const Layout L = make_layout(maxn, max_nnz, ring, arena, levels, hot_global);
             const uint32_t spec_min = spec_min_n();
             const bool spec = spec_min > 0 && cls_maxn[cls] >= spec_min;
             int per_cu = 0;
             GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls), kWave, 0));
             if (per_cu <= 0) per_cu = 1;
             const int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
+            // automorphism rings: only the worker that owns a graph stores
+            // automorphisms, so a class with fewer graphs than workers keeps
+            // one ring per graph in a pool (large rings: the reference-shaped
+            // entry points use max_aut rows of n)
+            const uint64_t ring_bytes = 2ull * ring * maxn;
+            const bool pool_rings = ring_bytes > (1ull << 20) && (int64_t)cnt < resident;
+            const Layout L = make_layout(maxn, max_nnz, ring, arena, levels, hot_global, pool_rings);
             size_t free_b = 0, total_b = 0;
             GCL_HIP(hipMemGetInfo(&free_b, &total_b));
-            const int64_t mem_workers = (int64_t)((double)free_b * 0.5 / (double)L.total);
+            const double pool_b = pool_rings ? (double)ring_bytes * (double)cnt : 0.0;
+            const int64_t mem_workers = (int64_t)(((double)free_b - pool_b) * 0.5 / (double)L.total);
             // with speculation every resident worker runs: the ones without a
             // graph refine published root children of the others
             const int64_t want = spec ? (int64_t)(cnt + cls_spec_work[cls]) : (int64_t)cnt;
@@ -325,6 +332,7 @@ struct DeviceBatch {
                                   hipMemcpyHostToDevice));
             }
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
+            if (pool_rings) GCL_HIP(hipMalloc(&C.autpool.p, (size_t)(ring_bytes * cnt)));
             if (!side.empty()) {
                 GCL_HIP(hipMalloc(&C.diag.p, sizeof(DiagRecord) * (size_t)C.grid));
                 GCL_HIP(hipMemset(C.diag.p, 0, sizeof(DiagRecord) * (size_t)C.grid));
@@ -365,6 +373,8 @@ struct DeviceBatch {
             P.board_mask = spec ? (unsigned long long*)C.board.p + (spec_records + 63) / 64 : nullptr;
             P.pending = (unsigned int*)ctr.p + 32 + cls;
             P.diag = side.empty() ? nullptr : (DiagRecord*)C.diag.p;
+            P.aut_pool = pool_rings ? (ix_t*)C.autpool.p : nullptr;
+            P.aut_stride = pool_rings ? ring_bytes / sizeof(ix_t) : 0;
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));

@@ -61,8 +61,10 @@ constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
 // that double as packed counters start on a 32-bit word
 __host__ __device__ inline uint64_t hot_stride(uint32_t maxn) { return ((uint64_t)maxn + 2 + 63) & ~uint64_t(63); }
 
+// pooled_auts: the automorphism rings live in a pool, one per graph of the
+// launch, instead of in each worker's scratch
 __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, uint32_t ring, uint64_t arena,
-                                              uint32_t levels, bool hot_global) {
+                                              uint32_t levels, bool hot_global, bool pooled_auts = false) {
     Layout L;
     uint64_t o = 0;
     const uint64_t n1 = (uint64_t)maxn + 2;
@@ -84,7 +86,7 @@ __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, u
     L.first_row = o; o = align256(o + 4 * n1);
     L.best_row = o; o = align256(o + 4 * n1);
     L.subgroup = o; o = align256(o + 4 * (uint64_t)ring);
-    L.auts = o; o = align256(o + 2 * (uint64_t)ring * maxn);
+    L.auts = o; if (!pooled_auts) o = align256(o + 2 * (uint64_t)ring * maxn);
     L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * hot_stride(maxn));
     L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (hot_stride(maxn) / 2 + 1));
     L.hot64 = o; if (hot_global) o = align256(o + 8ull * kHot64 * nw1);
@@ -190,6 +192,8 @@ struct KParams {
     unsigned int* pending;      // running graphs that may still publish
     uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
     DiagRecord* diag;           // per-worker records (grid entries), or NULL
+    ix_t* aut_pool;             // automorphism rings, one per item (aut_stride entries), or NULL
+    uint64_t aut_stride;
     ServeParams sv;             // serving mode when sv.tail != nullptr
 };
 
@@ -2295,7 +2299,7 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.first_row = (uint32_t*)(base + L.first_row);
     W.best_row = (uint32_t*)(base + L.best_row);
     W.subgroup = (uint32_t*)(base + L.subgroup);
-    W.auts = (ix_t*)(base + L.auts);
+    W.auts = P.aut_pool ? nullptr : (ix_t*)(base + L.auts);
     W.arena_cap = P.arena_cap;
     W.ring = P.ring;
     W.levels = L.levels;
@@ -2375,6 +2379,7 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             if (l == 0) item = atomicAdd(P.next_item, 1u);
             item = __shfl(item, 0, kWave);
             if (item < (uint32_t)P.num_items) {
+                if (P.aut_pool) W.auts = P.aut_pool + (uint64_t)item * P.aut_stride;
                 process_graph(W, P, P.items[item]);
                 continue;
             }
@@ -2538,7 +2543,7 @@ template <int MAXN>
 __global__ __launch_bounds__(kWave) void search_kernel_lds(KParams P) {
     __shared__ Worker<HotL<MAXN>> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false);
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false, P.aut_pool != nullptr);
     bind_scratch(W, P, base, L);
     worker_loop(W, P);
 }
@@ -2548,7 +2553,7 @@ template <int MAXN>
 __global__ __launch_bounds__(kWave) void serve_kernel_lds(KParams P) {
     __shared__ Worker<HotL<MAXN>> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false);
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false, P.aut_pool != nullptr);
     bind_scratch(W, P, base, L);
     serve_loop(W, P);
 }
@@ -2556,7 +2561,7 @@ __global__ __launch_bounds__(kWave) void serve_kernel_lds(KParams P) {
 __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
     __shared__ Worker<HotG> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true);
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true, P.aut_pool != nullptr);
     bind_scratch(W, P, base, L);
     W.H.h16 = (ix_t*)(base + L.hot16);
     W.H.h32 = (uint32_t*)(base + L.hot32);
@@ -2569,7 +2574,7 @@ __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
 __global__ __launch_bounds__(kWave) void serve_kernel_global(KParams P) {
     __shared__ Worker<HotG> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true);
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true, P.aut_pool != nullptr);
     bind_scratch(W, P, base, L);
     W.H.h16 = (ix_t*)(base + L.hot16);
     W.H.h32 = (uint32_t*)(base + L.hot32);

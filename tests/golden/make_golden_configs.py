@@ -1,14 +1,16 @@
 """Writes tests/golden/configs.json: the reference's own sequential binary
 (oracle/_ref/gpu_canon_label_ref) on graphs of the benchmark configurations'
 full sizes (3-regular and torus graphs with n = 4096, CFI graphs with
-n = 1000 and 2000, each with a relabeled copy and a double-edge-swap
+n = 1000, 2000 and 4000, each with a relabeled copy and a double-edge-swap
 variant where the configuration names one).  The outputs are large, so the
 fixture keeps their SHA-256 and length with the node and automorphism
 counts; the inputs are re-created from libgcl's seeded generators, pinned by
 the SHA-256 of their DIMACS text.  Takes about 10 minutes of one CPU core
 (the reference needs ~150 s per 3-regular graph).
 
-    python tests/golden/make_golden_configs.py      (needs the reference sources)
+    python tests/golden/make_golden_configs.py [NAME ...]   (needs the reference sources)
+
+With names, only those entries are (re)computed and the others are kept.
 """
 import hashlib
 import json
@@ -39,6 +41,8 @@ RECIPES = {
     "cfi100_s1_twisted": ["cfi", 100, 1, 1],
     "cfi200_s2_straight": ["cfi", 200, 0, 2],
     "cfi200_s2_twisted": ["cfi", 200, 1, 2],
+    "cfi400_s3_straight": ["cfi", 400, 0, 3],
+    "cfi400_s3_twisted": ["cfi", 400, 1, 3],
 }
 
 
@@ -61,12 +65,24 @@ def build_graph(name, cache=None):
     return g
 
 
-def main():
+def main(names):
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: run make -C oracle ref")
+    unknown = [x for x in names if x not in RECIPES]
+    if unknown:
+        sys.exit(f"unknown names: {unknown}")
+    path = os.path.join(HERE, "configs.json")
+    kept = {}
+    if names and os.path.exists(path):
+        with open(path) as fh:
+            kept = {e["name"]: e for e in json.load(fh)["graphs"]}
     cache, out = {}, []
     with tempfile.TemporaryDirectory() as tmp:
         for name in RECIPES:
+            if names and name not in names:
+                if name in kept:
+                    out.append(kept[name])
+                continue
             g = build_graph(name, cache)
             f = os.path.join(tmp, name + ".dimacs")
             gcl.write_dimacs(f, g)
@@ -81,10 +97,10 @@ def main():
                             automorphisms=int(RE_AUT.search(p.stdout).group(1)),
                             output_sha256=hashlib.sha256(data).hexdigest(), output_bytes=len(data)))
             print(name, out[-1]["nodes"], out[-1]["automorphisms"], flush=True)
-    with open(os.path.join(HERE, "configs.json"), "w") as fh:
+    with open(path, "w") as fh:
         json.dump(dict(source="reference sequential binary (oracle/_ref), --sequential --max-aut=10000",
                        graphs=out), fh, indent=1)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
