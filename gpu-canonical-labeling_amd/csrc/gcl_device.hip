@@ -298,7 +298,7 @@ struct DeviceBatch {
             // automorphisms, so a class with fewer graphs than workers keeps
             // one ring per graph in a pool (large rings: the reference-shaped
             // entry points use max_aut rows of n)
-            const uint64_t ring_bytes = 2ull * ring * maxn;
+            const uint64_t ring_bytes = aut_region_bytes(ring, maxn);
             const bool pool_rings = ring_bytes > (1ull << 20) && (int64_t)cnt < resident;
             const Layout L = make_layout(maxn, max_nnz, ring, arena, levels, hot_global, pool_rings);
             size_t free_b = 0, total_b = 0;
@@ -373,8 +373,8 @@ struct DeviceBatch {
             P.board_mask = spec ? (unsigned long long*)C.board.p + (spec_records + 63) / 64 : nullptr;
             P.pending = (unsigned int*)ctr.p + 32 + cls;
             P.diag = side.empty() ? nullptr : (DiagRecord*)C.diag.p;
-            P.aut_pool = pool_rings ? (ix_t*)C.autpool.p : nullptr;
-            P.aut_stride = pool_rings ? ring_bytes / sizeof(ix_t) : 0;
+            P.aut_pool = pool_rings ? (uint8_t*)C.autpool.p : nullptr;
+            P.aut_stride = pool_rings ? ring_bytes : 0;
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));

@@ -61,6 +61,13 @@ constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
 // that double as packed counters start on a 32-bit word
 __host__ __device__ inline uint64_t hot_stride(uint32_t maxn) { return ((uint64_t)maxn + 2 + 63) & ~uint64_t(63); }
 
+// Automorphism ring of one graph: per slot a fix length (below), then the
+// slots' images (ring x maxn entries).
+__host__ __device__ inline uint64_t aut_fix_bytes(uint32_t ring) { return align256(4 * (uint64_t)ring); }
+__host__ __device__ inline uint64_t aut_region_bytes(uint32_t ring, uint32_t maxn) {
+    return aut_fix_bytes(ring) + align256(2 * (uint64_t)ring * maxn);
+}
+
 // pooled_auts: the automorphism rings live in a pool, one per graph of the
 // launch, instead of in each worker's scratch
 __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, uint32_t ring, uint64_t arena,
@@ -86,7 +93,7 @@ __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, u
     L.first_row = o; o = align256(o + 4 * n1);
     L.best_row = o; o = align256(o + 4 * n1);
     L.subgroup = o; o = align256(o + 4 * (uint64_t)ring);
-    L.auts = o; if (!pooled_auts) o = align256(o + 2 * (uint64_t)ring * maxn);
+    L.auts = o; if (!pooled_auts) o += aut_region_bytes(ring, maxn);
     L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * hot_stride(maxn));
     L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (hot_stride(maxn) / 2 + 1));
     L.hot64 = o; if (hot_global) o = align256(o + 8ull * kHot64 * nw1);
@@ -192,7 +199,7 @@ struct KParams {
     unsigned int* pending;      // running graphs that may still publish
     uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
     DiagRecord* diag;           // per-worker records (grid entries), or NULL
-    ix_t* aut_pool;             // automorphism rings, one per item (aut_stride entries), or NULL
+    uint8_t* aut_pool;          // automorphism rings, one per item (aut_stride bytes), or NULL
     uint64_t aut_stride;
     ServeParams sv;             // serving mode when sv.tail != nullptr
 };
@@ -448,6 +455,7 @@ struct Worker {
     // per-worker global scratch
     uint64_t* snap;
     ix_t *seq, *first_seq, *best_seq, *arena_mem, *arena_rep, *first_label, *best_label, *auts;
+    uint32_t* aut_fix;  // per ring slot: how much of seq its automorphism fixes
     LevelRec* lv;
     int32_t *firstcode, *canon_code;
     uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
@@ -1472,19 +1480,32 @@ struct Worker {
         const LevelRec r = lv[li];
         const int64_t upto = min(num_aut, (int64_t)ring);
         const int64_t from = r.last_num_aut < 0 ? 0 : r.last_num_aut;
-        // ring slots whose automorphism fixes the sequence: one slot per lane
+        // ring slots whose automorphism fixes the sequence: one slot per lane;
+        // aut_fix[i] = 2k+1 when seq[k] is the first entry the slot moves,
+        // 2k when it fixes seq[0, k) and the rest is unchecked, so each seq
+        // entry is checked once per slot until backtracking rewrites it
+        const uint32_t ss = seq_size;
         uint32_t nsub = 0;
         for (int64_t base = from; base < upto; base += kWave) {
             const int64_t i = base + l;
-            bool ok = i < upto;
-            if (ok) {
-                const ix_t* a = auts + (uint64_t)i * n;
-                for (uint32_t j = 0; j < seq_size; ++j) {
-                    const uint32_t sj = seq[j];
-                    if (a[sj] != sj) {
-                        ok = false;
-                        break;
+            bool ok = false;
+            if (i < upto) {
+                uint32_t x = aut_fix[i];
+                if ((x >> 1) >= ss) {
+                    ok = true;
+                } else if (!(x & 1u)) {
+                    const ix_t* a = auts + (uint64_t)i * n;
+                    ok = true;
+                    for (uint32_t j = x >> 1; j < ss; ++j) {
+                        const uint32_t sj = seq[j];
+                        if (a[sj] != sj) {
+                            ok = false;
+                            x = (j << 1) | 1u;
+                            break;
+                        }
                     }
+                    if (ok) x = ss << 1;
+                    aut_fix[i] = x;
                 }
             }
             const uint64_t m = ballot(ok);
@@ -1493,48 +1514,56 @@ struct Worker {
         }
         group_sync();
         if (nsub == 0) return;
-        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt_st(v, v);  // cnt is zero between steps
+        // An automorphism that fixes the sequence maps the node's partition
+        // to itself (refinement is label-invariant), so it permutes the
+        // target cell: the orbits of the cell's members are the components of
+        // v -> a(v) over the members alone.  The union-find lives in cnt
+        // (zero between steps) at the members' entries, seeded with the
+        // representatives of the automorphisms already applied at this level.
+        const uint32_t t = r.tc_len, off = r.arena_off, tf = r.tc_first;
+        for (uint32_t i = (uint32_t)l; i < t; i += kWave) H.cnt_st(arena_mem[off + i], arena_rep[off + i]);
         H.sync();
-        for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave)
-            H.cnt_st(arena_mem[r.arena_off + i], arena_rep[r.arena_off + i]);
-        H.sync();
-        for (uint32_t s = 0; s < nsub; ++s) {
-            const ix_t* a = auts + (uint64_t)subgroup[s] * n;
-            for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.sb(v) = a[v];  // stage the images
-            H.sync();
-            for (uint32_t v = (uint32_t)l; v < n; v += kWave) {
-                const uint32_t img = H.sb(v);
-                if (img == v) continue;
-                // parents only point to smaller ids, so walks are shorter
-                // than n; the caps only bound a corrupted structure
-                uint32_t x = v, y = img;
-                for (uint32_t attempt = 0; attempt < n; ++attempt) {
-                    for (uint32_t h = 0; h < n; ++h) {
-                        const uint32_t px = H.cnt_ld(x);
-                        if (px == x) break;
-                        x = px;
-                    }
-                    for (uint32_t h = 0; h < n; ++h) {
-                        const uint32_t py = H.cnt_ld(y);
-                        if (py == y) break;
-                        y = py;
-                    }
-                    if (x == y) break;
-                    const uint32_t lo = min(x, y), hi = max(x, y);
-                    if (H.cnt_cas(hi, hi, lo)) break;
-                    x = lo;
-                    y = hi;
-                }
+        bool outside = false;
+        const uint64_t pairs = (uint64_t)nsub * t;
+        for (uint64_t k = (uint64_t)l; k < pairs; k += kWave) {
+            const uint32_t si = (uint32_t)(k / t), i = (uint32_t)(k - (uint64_t)si * t);
+            const uint32_t v = arena_mem[off + i];
+            const uint32_t img = auts[(uint64_t)subgroup[si] * n + v];
+            if (img == v) continue;
+            if ((uint32_t)H.pos(img) - tf >= t) {
+                outside = true;
+                continue;
             }
-            H.sync();
-        }
-        for (uint32_t i = (uint32_t)l; i < r.tc_len; i += kWave) {
-            uint32_t x = arena_mem[r.arena_off + i];
-            for (uint32_t h = 0; h < n && H.cnt_ld(x) != x; ++h) x = H.cnt_ld(x);
-            arena_rep[r.arena_off + i] = (ix_t)x;
+            // parents only point to smaller ids, so walks are shorter than
+            // the cell; the caps only bound a corrupted structure
+            uint32_t x = v, y = img;
+            for (uint32_t attempt = 0; attempt < n; ++attempt) {
+                for (uint32_t h = 0; h < n; ++h) {
+                    const uint32_t px = H.cnt_ld(x);
+                    if (px == x) break;
+                    x = px;
+                }
+                for (uint32_t h = 0; h < n; ++h) {
+                    const uint32_t py = H.cnt_ld(y);
+                    if (py == y) break;
+                    y = py;
+                }
+                if (x == y) break;
+                const uint32_t lo = min(x, y), hi = max(x, y);
+                if (H.cnt_cas(hi, hi, lo)) break;
+                x = lo;
+                y = hi;
+            }
         }
         H.sync();
-        for (uint32_t v = (uint32_t)l; v < n; v += kWave) H.cnt_st(v, 0);
+        for (uint32_t i = (uint32_t)l; i < t; i += kWave) {
+            uint32_t x = arena_mem[off + i];
+            for (uint32_t h = 0; h < n && H.cnt_ld(x) != x; ++h) x = H.cnt_ld(x);
+            arena_rep[off + i] = (ix_t)x;
+        }
+        H.sync();
+        for (uint32_t i = (uint32_t)l; i < t; i += kWave) H.cnt_st(arena_mem[off + i], 0);
+        if (ballot(outside)) status = GCL_E_INTERNAL;  // not an automorphism of this node
         group_sync();
     }
 
@@ -1685,11 +1714,31 @@ struct Worker {
         return 0;
     }
 
+    __device__ void bind_auts(uint8_t* region) {
+        aut_fix = (uint32_t*)region;
+        auts = region ? (ix_t*)(region + aut_fix_bytes(ring)) : nullptr;
+    }
+
     __device__ void store_aut(const ix_t* ref_label) {
         const uint64_t slot = (uint64_t)(num_aut % (int64_t)ring);
         ix_t* dst = auts + slot * n;
         for (uint32_t j = (uint32_t)lane_id(); j < n; j += kWave) dst[ref_label[j]] = H.elem(j);
+        if (lane_id() == 0) aut_fix[slot] = 0;  // nothing of seq checked yet
         ++num_aut;
+        group_sync();
+    }
+
+    // The sequence shrank to seq_size entries: fix lengths beyond it (or a
+    // mismatch at position seq_size, whose entry will be rewritten) go back
+    // to "matches the first seq_size entries, rest unchecked".
+    __device__ void truncate_fix() {
+        const int64_t cnt = min(num_aut, (int64_t)ring);
+        if (cnt == 0) return;
+        const uint32_t keep = seq_size << 1;
+        for (int64_t i = lane_id(); i < cnt; i += kWave) {
+            const uint32_t x = aut_fix[i];
+            if (x > keep) aut_fix[i] = keep;
+        }
         group_sync();
     }
 
@@ -1718,6 +1767,7 @@ struct Worker {
             comp_canon = 0;
         }
         seq_size = lev - 1;
+        truncate_fix();
         tc_level_reached = lev;
         level = lev;
     }
@@ -2299,9 +2349,9 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.first_row = (uint32_t*)(base + L.first_row);
     W.best_row = (uint32_t*)(base + L.best_row);
     W.subgroup = (uint32_t*)(base + L.subgroup);
-    W.auts = P.aut_pool ? nullptr : (ix_t*)(base + L.auts);
     W.arena_cap = P.arena_cap;
     W.ring = P.ring;
+    W.bind_auts(P.aut_pool ? nullptr : base + L.auts);
     W.levels = L.levels;
     W.step_budget = P.step_budget;
     const uint64_t now = wall_clock64();
@@ -2379,7 +2429,7 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             if (l == 0) item = atomicAdd(P.next_item, 1u);
             item = __shfl(item, 0, kWave);
             if (item < (uint32_t)P.num_items) {
-                if (P.aut_pool) W.auts = P.aut_pool + (uint64_t)item * P.aut_stride;
+                if (P.aut_pool) W.bind_auts(P.aut_pool + (uint64_t)item * P.aut_stride);
                 process_graph(W, P, P.items[item]);
                 continue;
             }

@@ -73,10 +73,12 @@ def test_gpu_config_outputs_match_reference():
         sha[e["name"]] = e["output_sha256"]
     # verdicts of the configurations' pairs follow from the reference outputs
     for base in ("rr3_n4096_s0", "torus64x64"):
+        print(f"config pairs {base}", flush=True)
         assert gcl.is_isomorphic(get(base), get(base + "_perm"), engine="gpu") == (sha[base] == sha[base + "_perm"])
         assert gcl.is_isomorphic(get(base), get(base + "_swap"), engine="gpu") == (sha[base] == sha[base + "_swap"])
     cfi = sorted({k[:-len("_straight")] for k in sha if k.endswith("_straight")})
     assert cfi, "no CFI pairs in the fixture"
     for base in cfi:
+        print(f"config pair {base}", flush=True)
         assert gcl.is_isomorphic(get(base + "_straight"), get(base + "_twisted"), engine="gpu") == (
             sha[base + "_straight"] == sha[base + "_twisted"])
