@@ -456,6 +456,7 @@ struct Worker {
     uint64_t* snap;
     ix_t *seq, *first_seq, *best_seq, *arena_mem, *arena_rep, *first_label, *best_label, *auts;
     uint32_t* aut_fix;  // per ring slot: how much of seq its automorphism fixes
+    uint32_t fix_low;   // lowest seq_size since aut_fix was last truncated
     LevelRec* lv;
     int32_t *firstcode, *canon_code;
     uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
@@ -1484,6 +1485,7 @@ struct Worker {
         // aut_fix[i] = 2k+1 when seq[k] is the first entry the slot moves,
         // 2k when it fixes seq[0, k) and the rest is unchecked, so each seq
         // entry is checked once per slot until backtracking rewrites it
+        truncate_fix();
         const uint32_t ss = seq_size;
         uint32_t nsub = 0;
         for (int64_t base = from; base < upto; base += kWave) {
@@ -1524,9 +1526,13 @@ struct Worker {
         for (uint32_t i = (uint32_t)l; i < t; i += kWave) H.cnt_st(arena_mem[off + i], arena_rep[off + i]);
         H.sync();
         bool outside = false;
-        const uint64_t pairs = (uint64_t)nsub * t;
-        for (uint64_t k = (uint64_t)l; k < pairs; k += kWave) {
-            const uint32_t si = (uint32_t)(k / t), i = (uint32_t)(k - (uint64_t)si * t);
+        // (automorphism, member) pairs, one per lane; 32-bit indices (t <= n
+        // < 2^16, so a chunk of 2^15 automorphisms stays below 2^31 pairs)
+        for (uint32_t s0 = 0; s0 < nsub; s0 += 32768u) {
+          const uint32_t pairs = min(nsub - s0, 32768u) * t;
+          for (uint32_t k = (uint32_t)l; k < pairs; k += kWave) {
+            const uint32_t q = k / t;
+            const uint32_t si = s0 + q, i = k - q * t;
             const uint32_t v = arena_mem[off + i];
             const uint32_t img = auts[(uint64_t)subgroup[si] * n + v];
             if (img == v) continue;
@@ -1554,6 +1560,7 @@ struct Worker {
                 x = lo;
                 y = hi;
             }
+          }
         }
         H.sync();
         for (uint32_t i = (uint32_t)l; i < t; i += kWave) {
@@ -1728,13 +1735,17 @@ struct Worker {
         group_sync();
     }
 
-    // The sequence shrank to seq_size entries: fix lengths beyond it (or a
-    // mismatch at position seq_size, whose entry will be rewritten) go back
-    // to "matches the first seq_size entries, rest unchecked".
+    // The sequence shrank to fix_low entries at some point since the fix
+    // lengths were last brought up to date (backtracking only records the
+    // lowest length; truncations to s1, then s2 act as one to min(s1, s2)):
+    // lengths beyond it, or a mismatch at an entry since rewritten, go back
+    // to "matches the first fix_low entries, rest unchecked".
     __device__ void truncate_fix() {
         const int64_t cnt = min(num_aut, (int64_t)ring);
-        if (cnt == 0) return;
-        const uint32_t keep = seq_size << 1;
+        const uint32_t low = fix_low;
+        fix_low = 0xFFFFFFFFu;
+        if (cnt == 0 || low == 0xFFFFFFFFu) return;
+        const uint32_t keep = low << 1;
         for (int64_t i = lane_id(); i < cnt; i += kWave) {
             const uint32_t x = aut_fix[i];
             if (x > keep) aut_fix[i] = keep;
@@ -1767,7 +1778,7 @@ struct Worker {
             comp_canon = 0;
         }
         seq_size = lev - 1;
-        truncate_fix();
+        fix_low = min(fix_low, seq_size);
         tc_level_reached = lev;
         level = lev;
     }
@@ -2174,6 +2185,7 @@ struct Worker {
         budget = step_budget;
         spec_slot = -1;
         root_changed = false;
+        fix_low = 0xFFFFFFFFu;
         spec_maxn = P.maxn;
         // counted in *pending by the host before the launch
         spec_pending = P.spec_min_n > 0 && n >= P.spec_min_n;
