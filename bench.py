@@ -99,9 +99,7 @@ def cpu_baseline(batch, budget_s, stride):
 def served(args):
     """Served measurement: requests over the same mix of sizes, half
     canonicalize and half isomorphism tests (partner: a permuted copy or a
-    one-swap variant, 50/50), through the request server.  Phase 1 submits
-    them all at once (capacity); phase 2 replays a Poisson stream at half that
-    rate (latency from arrival to completion)."""
+    one-swap variant, 50/50), through the request server."""
     R = args.serve_requests
     t0 = time.time()
     first = gcl.mix_batch(R, args.seed + 1)
@@ -115,9 +113,58 @@ def served(args):
             partners.append(gcl.permute(g, 1000003 + i) if rs.rand() < 0.5 else gcl.edge_swap(g, 2000003 + i))
             pair_with[i] = R + len(partners) - 1
     batch = gcl.Batch(graphs + partners)
-    labelings = R + len(partners)
     log(f"served workload: {R} requests ({len(partners)} pairs) built in {time.time() - t0:.1f}s")
-    srv = gcl.Server(max_n=1024, max_aut=10000, lifetime_ms=600000)
+    out = serve_run(args, batch, R, pair_with, len(partners), "served", dict(max_n=1024))
+    out["workload"] = ("same size mix (n log-uniform [16,1024]); half canonicalize, half pair tests "
+                       "(permuted copy or one-swap variant, 50/50); persistent serving kernels, one per size "
+                       "class; the 2% of n 10k-50k graphs of the serving configuration are in served_large")
+    return out
+
+
+def served_large(args):
+    """The serving configuration's 2% of large graphs: requests over the same
+    size mix where each is, with probability 0.02, a sparse G(n, p) of mean
+    degree 8 with n log-uniform in [10000, 50000]; half canonicalize, half
+    pair tests as above.  The server runs a second persistent kernel for the
+    large class (global-memory partition state)."""
+    R = args.serve_large_requests
+    t0 = time.time()
+    first = gcl.mix_batch(R, args.seed + 11)
+    rs = np.random.RandomState(args.seed + 12)
+    graphs, nlarge = [], 0
+    for i in range(R):
+        if rs.rand() < 0.02:
+            n = int(round(np.exp(rs.uniform(np.log(10000), np.log(50000)))))
+            graphs.append(gcl.erdos_renyi_mean_degree(n, 8.0, 3000017 + i))
+            nlarge += 1
+        else:
+            graphs.append(first.graph(i))
+    pair_with = np.full(R, -1, np.int32)
+    partners = []
+    for i in range(R):
+        if rs.rand() < 0.5:
+            g = graphs[i]
+            partners.append(gcl.permute(g, 4000037 + i) if rs.rand() < 0.5 else gcl.edge_swap(g, 5000011 + i))
+            pair_with[i] = R + len(partners) - 1
+    batch = gcl.Batch(graphs + partners)
+    max_nnz = max(int(g.nnz) for g in graphs + partners)
+    log(f"served-large workload: {R} requests ({nlarge} large, {len(partners)} pairs) built in "
+        f"{time.time() - t0:.1f}s")
+    out = serve_run(args, batch, R, pair_with, len(partners), "served-large",
+                    dict(max_n=50000, max_graph_nnz=max_nnz))
+    out["large_requests"] = nlarge
+    out["workload"] = ("same size mix with 2% sparse G(n,p) of mean degree 8, n log-uniform [10000,50000]; "
+                       "half canonicalize, half pair tests (permuted copy or one-swap variant, 50/50); the "
+                       "large 3-regular graphs of the serving configuration are not in this bounded run "
+                       "(n + 1 search nodes and about n^2 splitter steps each)")
+    return out
+
+
+def serve_run(args, batch, R, pair_with, npairs, tag, cfg):
+    """Phase 1 submits the R requests at once (capacity); phase 2 replays a
+    Poisson stream at half that rate (latency from arrival to completion)."""
+    labelings = R + npairs
+    srv = gcl.Server(max_aut=10000, lifetime_ms=600000, **cfg)
     try:
         srv.replay(batch, np.zeros(min(R, 512)), pair_with[:min(R, 512)])  # warm-up
         s0 = srv.stats()
@@ -139,10 +186,10 @@ def served(args):
     finally:
         srv.close()
     lat = r2["latency_ms"]
-    log(f"served: capacity {cap:.0f} labelings/s ({cap_s:.2f}s), poisson {rate:.0f} req/s: "
+    log(f"{tag}: capacity {cap:.0f} labelings/s ({cap_s:.2f}s), poisson {rate:.0f} req/s: "
         f"p50 {np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms")
     return {"value": round(cap, 2), "unit": "labelings/s",
-            "requests": R, "pair_requests": len(partners),
+            "requests": R, "pair_requests": npairs,
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
             "poisson_rate_req_s": round(rate, 1), "poisson_wall_s": round(lat_s, 3),
@@ -151,11 +198,7 @@ def served(args):
                           "p99": round(float(np.percentile(r2["device_ms"], 99)), 3)},
             "capacity_scheduler_ms": {k: round(s1["sched_" + k + "_ms"] - s0["sched_" + k + "_ms"], 1)
                                       for k in ("publish", "collect")},
-            "serving_kernels": int(st["launches"]),
-            "workload": "same size mix (n log-uniform [16,1024]); half canonicalize, half pair tests "
-                        "(permuted copy or one-swap variant, 50/50); persistent serving kernels, one per size "
-                        "class; the 2% of n 10k-50k graphs of the serving configuration are not in this "
-                        "bounded run"}
+            "serving_kernels": int(st["launches"])}
 
 
 def main():
@@ -172,6 +215,8 @@ def main():
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the served measurement (0 skips it)")
+    ap.add_argument("--serve-large-requests", type=int, default=2000,
+                    help="requests of the served measurement with 2%% large graphs (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -271,6 +316,8 @@ def main():
             out["served"] = served(args)
             out["served_vs_plain"] = round(out["served"]["value"] / plain_rate, 2)
             out["p99_latency_ms"] = out["served"]["latency_ms"]["p99"]
+        if args.serve_large_requests > 0:
+            out["served_large"] = served_large(args)
         out["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget, 128)
     print(json.dumps(out), flush=True)
 

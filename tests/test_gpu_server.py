@@ -149,3 +149,39 @@ def test_server_golden_graphs_and_pairs(golden):
             assert ("ISOMORPHIC" if r["verdict"] == 1 else "NOT ISOMORPHIC") == p["verdict"], p
     finally:
         srv.close()
+
+
+def test_server_large_class_same_as_batch_path():
+    # graphs of n 10k-50k go to the server's second (global-memory) serving
+    # kernel; alone and amid small requests they get the batch path's labels,
+    # node counts and pair verdicts
+    big = [gcl.permute(gcl.erdos_renyi_mean_degree(n, 8.0, 40 + n), n) for n in (12000, 30000)]
+    ref = [gcl.canon_label(g, engine="gpu") for g in big]
+    swap = gcl.edge_swap(big[0], 9)
+    srv = gcl.Server(max_n=50000, max_graph_nnz=max(g.nnz for g in big + [swap]), lifetime_ms=60000)
+    try:
+        for g, r in zip(big, ref):
+            _log(f"large alone n={g.n}")
+            t = srv.submit(g)
+            res = srv.wait_all([t], timeout_s=60)[t]
+            assert res["status"] == 0
+            assert np.array_equal(res["labels"], r.label) and res["nodes"] == r.nodes
+        _log("large in a mix")
+        small = _mix(300, 41)
+        tickets, want = [], {}
+        for i, g in enumerate(small):
+            tickets.append(srv.submit(g))
+            if i == 100:
+                t = srv.submit(big[1])
+                tickets.append(t)
+                want[t] = ref[1]
+            if i == 200:
+                tp = srv.submit(big[0], swap)
+                tickets.append(tp)
+        got = srv.wait_all(tickets, timeout_s=90)
+        assert all(got[t]["status"] == 0 for t in tickets)
+        for t, r in want.items():
+            assert np.array_equal(got[t]["labels"], r.label) and got[t]["nodes"] == r.nodes
+        assert got[tp]["verdict"] == int(gcl.is_isomorphic(big[0], swap, engine="gpu"))
+    finally:
+        srv.close()
