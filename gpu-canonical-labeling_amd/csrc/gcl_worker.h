@@ -243,6 +243,13 @@ __device__ __forceinline__ uint32_t wave_scan_max(uint32_t x) {  // inclusive ma
     return x;
 }
 // reductions: the result is wave-uniform (a scalar register)
+// A copy of x in a vector register: arithmetic on it stays on the vector
+// unit even though x is the same in every lane.
+__device__ __forceinline__ uint32_t to_vgpr(uint32_t x) {
+    uint32_t v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) { return readlane(wave_scan(x), kWave - 1); }
 __device__ __forceinline__ uint32_t wave_max(uint32_t x) { return readlane(wave_scan_max(x), kWave - 1); }
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) { return ~wave_max(~x); }
@@ -984,18 +991,29 @@ struct Worker {
         }
         X.nactive += (uint32_t)uni((int32_t)wave_sum((uint32_t)dact));
         H.sync();
+        // The MASH chain over hits[0, nval) in order.  Each MASH's mask is
+        // deferred to the end (the low 15 bits of (a ^ C) + x depend only on
+        // the low 15 bits of a and x), and the chain runs in a vector
+        // register, equal in all lanes: one lane read and one xor-add per
+        // value, with no round trip through the scalar unit.
+        uint32_t lc = to_vgpr(longcode);
+        const uint32_t kx = to_vgpr(065435u);
         for (uint32_t base = 0; base < nval; base += kWave) {
             const uint32_t x = base + (uint32_t)l < nval ? H.hits_ld(base + (uint32_t)l) : 0u;
             const uint32_t here = min((uint32_t)kWave, nval - base);
-            uint32_t t = 0;
-            for (; t + 4 <= here; t += 4) {
-                longcode = GCL_MASH(longcode, readlane(x, (int)t));
-                longcode = GCL_MASH(longcode, readlane(x, (int)t + 1));
-                longcode = GCL_MASH(longcode, readlane(x, (int)t + 2));
-                longcode = GCL_MASH(longcode, readlane(x, (int)t + 3));
+            if (here == (uint32_t)kWave) {
+#pragma unroll
+                for (int u = 0; u < kWave; ++u) lc = (lc ^ kx) + readlane(x, u);
+            } else {
+                uint32_t t = 0;
+                for (; t + 8 <= here; t += 8) {
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) lc = (lc ^ kx) + readlane(x, (int)(t + u));
+                }
+                for (; t < here; ++t) lc = (lc ^ kx) + readlane(x, (int)t);
             }
-            for (; t < here; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
         }
+        longcode = uni(lc) & 077777u;
         H.sync();
         for (uint32_t b = (uint32_t)l; b < nval; b += kWave) H.hits_st(b, 0);
     }
