@@ -1220,7 +1220,7 @@ struct Worker {
         }
         H.sync();
         // hash chain / active rules in position order (reference 253-468)
-        if (Hot::kLds && s > 1) {
+        if (s > 1) {
             hash_pass_parallel(X, nns, longcode, hint);
         } else
         for (uint32_t base = 0; base < nns; base += kWave) {
@@ -1297,7 +1297,7 @@ struct Worker {
         // cell masks of the new fragments; clear the split marks.  The
         // marks are keyed by old first positions: walk the ns list again
         // (done by the parallel hash pass when it ran)
-        if (!(Hot::kLds && s > 1))
+        if (s == 1)
         for (uint32_t base = 0; base < nns; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             if (i < nns) {
