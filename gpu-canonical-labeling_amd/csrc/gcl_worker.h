@@ -649,14 +649,14 @@ struct Worker {
     // Refinement step, splitter {w} with deg(w) <= 64 (reference
     // src/partition_refinement.cpp:253-468, splitter.length == 1): cells
     // holding neighbours split into (neighbours, others).
-    __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t w, uint32_t e0, uint32_t deg, uint32_t& longcode,
-                                                   int& hint) {
+    // u: this lane's neighbour of the splitter (lanes below deg), already
+    // loaded by the caller
+    __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t u, uint32_t deg, uint32_t& longcode, int& hint) {
         const int l = lane_id();
         X.csr_bytes += 12 + 4ull * deg;
-        uint32_t u = 0, c = 0, f = 0, r = 0;
+        uint32_t c = 0, f = 0, r = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
-            u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
             c = H.cend(H.pos(u));
             f = H.cfirst(c);
             ns = f != c;
@@ -1385,18 +1385,20 @@ struct Worker {
             --X.nactive;
             ++X.steps;
             const uint32_t e = uni((uint32_t)H.cend(p));
+            // a singleton splitter's neighbour ids are requested before the
+            // sync, so that the load overlaps it and the bookkeeping
+            uint32_t w = 0, e0 = 0, deg = 0, u = 0;
+            if (e == p) {
+                w = uni((uint32_t)H.elem(p));
+                e0 = uni(xrow(X, w));
+                deg = uni(xrow(X, w + 1)) - e0;
+                if (deg <= (uint32_t)kWave && (uint32_t)l < deg) u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
+            }
             longcode = GCL_MASH(longcode, p + e);
             if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
             H.sync();
-            if (e == p) {
-                const uint32_t w = uni((uint32_t)H.elem(p));
-                const uint32_t e0 = uni(xrow(X, w));
-                const uint32_t deg = uni(xrow(X, w + 1)) - e0;
-                if (deg <= (uint32_t)kWave) singleton_step(X, w, e0, deg, longcode, hint);
-                else general_step(X, p, e, longcode, hint);
-            } else {
-                general_step(X, p, e, longcode, hint);
-            }
+            if (e == p && deg <= (uint32_t)kWave) singleton_step(X, u, deg, longcode, hint);
+            else general_step(X, p, e, longcode, hint);
         }
         budget = bud;
         if (X.nactive) {

@@ -158,6 +158,9 @@ def test_server_large_class_same_as_batch_path():
     big = [gcl.permute(gcl.erdos_renyi_mean_degree(n, 8.0, 40 + n), n) for n in (12000, 30000)]
     ref = [gcl.canon_label(g, engine="gpu") for g in big]
     swap = gcl.edge_swap(big[0], 9)
+    # batch-path results first: a batch launched while serving kernels hold
+    # the GPU waits for them to exit
+    want_verdict = int(gcl.is_isomorphic(big[0], swap, engine="gpu"))
     srv = gcl.Server(max_n=50000, max_graph_nnz=max(g.nnz for g in big + [swap]), lifetime_ms=60000)
     try:
         for g, r in zip(big, ref):
@@ -182,6 +185,6 @@ def test_server_large_class_same_as_batch_path():
         assert all(got[t]["status"] == 0 for t in tickets)
         for t, r in want.items():
             assert np.array_equal(got[t]["labels"], r.label) and got[t]["nodes"] == r.nodes
-        assert got[tp]["verdict"] == int(gcl.is_isomorphic(big[0], swap, engine="gpu"))
+        assert got[tp]["verdict"] == want_verdict
     finally:
         srv.close()
