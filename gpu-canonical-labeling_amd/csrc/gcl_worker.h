@@ -524,11 +524,15 @@ struct Worker {
         }
     }
     // first set bit at position >= start, or -1
-    __device__ __forceinline__ static int32_t first_set_from(const uint64_t* m, uint32_t start, uint32_t nw) {
+    // First set bit at or after `start`, or -1; *wordp (if given) receives
+    // the whole bitmap word that holds it.
+    __device__ __forceinline__ static int32_t first_set_from(const uint64_t* m, uint32_t start, uint32_t nw,
+                                                             uint64_t* wordp = nullptr) {
         const int l = lane_id();
         for (uint32_t base = start >> 6; base < nw; base += kWave) {
             const uint32_t w = base + (uint32_t)l;
-            uint64_t bits = (w < nw) ? m[w] : 0ull;
+            const uint64_t raw = (w < nw) ? m[w] : 0ull;
+            uint64_t bits = raw;
             if (w == (start >> 6)) bits &= ~0ull << (start & 63);
             const uint64_t b = ballot(bits != 0ull);
             if (b) {
@@ -536,6 +540,8 @@ struct Worker {
                 const uint32_t lo = readlane((uint32_t)bits, fl);
                 const uint32_t hi = readlane((uint32_t)(bits >> 32), fl);
                 const uint64_t word = ((uint64_t)hi << 32) | lo;
+                if (wordp)
+                    *wordp = ((uint64_t)readlane((uint32_t)(raw >> 32), fl) << 32) | readlane((uint32_t)raw, fl);
                 return uni((int32_t)((base + (uint32_t)fl) * 64u + (uint32_t)__builtin_ctzll(word)));
             }
         }
@@ -1372,24 +1378,31 @@ struct Worker {
             // splitter: the first active position at or after the hint, else
             // the first one; the hint's own word is tested first (one scalar load)
             const uint32_t hw = (uint32_t)hint >> 6;
-            const uint64_t near = uni(H.active()[hw]) & (~0ull << ((uint32_t)hint & 63u));
+            const uint64_t hword = uni(H.active()[hw]);
+            const uint64_t near = hword & (~0ull << ((uint32_t)hint & 63u));
             int32_t sp;
+            uint64_t word = hword;  // the active word holding the splitter
             if (near) {
                 sp = (int32_t)(hw * 64u + (uint32_t)__builtin_ctzll(near));
             } else {
-                sp = first_set_from(H.active(), (hw + 1) * 64u, X.nw);
-                if (sp < 0) sp = first_set_from(H.active(), 0, X.nw);
+                sp = first_set_from(H.active(), (hw + 1) * 64u, X.nw, &word);
+                if (sp < 0) sp = first_set_from(H.active(), 0, X.nw, &word);
             }
             const uint32_t p = uni((uint32_t)sp);
-            set_bit_u(H.active(), p, false);
+            // clear its bit from the word in hand (no other lane writes the
+            // bitmap between steps)
+            if (l == 0) H.active()[p >> 6] = uni(word) & ~(1ull << (p & 63u));
             --X.nactive;
             ++X.steps;
+            // the cell end and the splitter's vertex (used when the cell is a
+            // singleton) are independent loads
             const uint32_t e = uni((uint32_t)H.cend(p));
+            const uint32_t wp = uni((uint32_t)H.elem(p));
             // a singleton splitter's neighbour ids are requested before the
             // sync, so that the load overlaps it and the bookkeeping
             uint32_t w = 0, e0 = 0, deg = 0, u = 0;
             if (e == p) {
-                w = uni((uint32_t)H.elem(p));
+                w = wp;
                 e0 = uni(xrow(X, w));
                 deg = uni(xrow(X, w + 1)) - e0;
                 if (deg <= (uint32_t)kWave && (uint32_t)l < deg) u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
