@@ -738,6 +738,14 @@ struct Worker {
         // boundary hash chain in position order (399-409): x = f+k-1 grows
         // with f; the hint is the candidate of the last split that has one
         const uint32_t x = f + k - 1;
+        if (nsplit == 1) {
+            const int j = __builtin_ctzll(om);
+            longcode = GCL_MASH(longcode, readlane(x, j));
+            const uint32_t hj = readlane(hkey, j);
+            if (hj != kNoKey) hint = (int)(hj & 0xFFFFu);
+            H.sync();
+            return;
+        }
         uint32_t rank = 0;
         for (uint64_t m = om; m; m &= m - 1) {
             const uint32_t xj = readlane(x, __builtin_ctzll(m));
