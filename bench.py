@@ -248,6 +248,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="pick the launch tuning with gcl_batch_autotune on a sample of this many graphs "
+                         "(untimed, before the warmup); 0 uses the defaults")
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the served measurement (0 skips it)")
     ap.add_argument("--serve-large-requests", type=int, default=2000,
@@ -265,7 +268,14 @@ def main():
     G = batch.num_graphs
     log(f"generated {G} graphs, n total {int(batch.vtx_off[-1])}, nnz {int(batch.row_ptr[-1])} "
         f"in {time.time() - t0:.1f}s")
-    db = gcl.DeviceBatch(batch, max_aut=10000)
+    tuning, tune_info = gcl.batch_tuning_default(), None
+    if args.autotune > 0:
+        t1 = time.time()
+        tuning, tms, tried = gcl.autotune(batch, max_aut=10000, sample=args.autotune)
+        tune_info = {"sample_graphs": args.autotune, "candidates": tried, "sample_kernel_ms": round(tms, 3),
+                     "seconds": round(time.time() - t1, 1)}
+        log(f"autotune: {tuning} ({tms:.1f} ms on the sample, {tried} candidates, {time.time() - t1:.1f}s)")
+    db = gcl.DeviceBatch(batch, max_aut=10000, tuning=tuning)
     use_graph = args.hipgraph
     for w in range(args.warmup):
         k, t = db.run(use_graph=use_graph)
@@ -311,7 +321,8 @@ def main():
         "data": "synthetic (seeded generators)",
         "config": {"workload": f"batched mix: {G} graphs, n log-uniform [16,1024], 50% G(n,p) deg U[3,12], "
                                "30% random d-regular d in {3,4,5}, 20% C_a x C_b a,b in U[4,32], seeded relabeling",
-                   "graphs": G, "seed": args.seed, "hipgraph": use_graph, "workers": res.workers},
+                   "graphs": G, "seed": args.seed, "hipgraph": use_graph, "workers": res.workers,
+                   "tuning": tuning, "autotune": tune_info},
         "search_nodes_per_s": round(res.total_nodes / (kernel_ms / 1e3), 1),
         "graph_search_ms": {"p50": round(float(np.percentile(search_ms, 50)), 3),
                             "p99": round(float(np.percentile(search_ms, 99)), 3),

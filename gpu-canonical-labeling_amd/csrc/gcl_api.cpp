@@ -267,6 +267,48 @@ gcl_status gcl_device_batch_create(const gcl_batch* b, uint32_t max_aut, gcl_dev
     return GCL_OK;
 }
 
+gcl_status gcl_batch_tuning_default(gcl_batch_tuning* t) {
+    if (!t) return fail(GCL_E_INVALID, "null tuning");
+    const BatchTuning d = default_batch_tuning();
+    t->spec_min_n = (int32_t)d.spec_min_n;
+    t->min_class_n = (int32_t)d.min_class_n;
+    return GCL_OK;
+}
+
+gcl_status gcl_device_batch_create_tuned(const gcl_batch* b, uint32_t max_aut, const gcl_batch_tuning* t,
+                                         gcl_device_batch** out) {
+    if (!out) return fail(GCL_E_INVALID, "null output handle");
+    gcl_status s = validate_batch(b);
+    if (s != GCL_OK) return s;
+    if ((s = check_max_aut(max_aut)) != GCL_OK) return s;
+    BatchTuning bt = default_batch_tuning();
+    if (t) {
+        if (t->spec_min_n < 0 || t->min_class_n < 1) return fail(GCL_E_INVALID, "negative tuning field");
+        bt.spec_min_n = (uint32_t)t->spec_min_n;
+        bt.min_class_n = (uint32_t)t->min_class_n;
+    }
+    DeviceBatch* db = nullptr;
+    s = device_batch_create(*b, max_aut, kBatchRing, &db, &bt);
+    if (s != GCL_OK) return s;
+    *out = reinterpret_cast<gcl_device_batch*>(db);
+    return GCL_OK;
+}
+
+gcl_status gcl_batch_autotune(const gcl_batch* b, uint32_t max_aut, int32_t sample, gcl_batch_tuning* best,
+                              double* best_ms, int32_t* tried) {
+    if (!best) return fail(GCL_E_INVALID, "null tuning");
+    gcl_status s = validate_batch(b);
+    if (s != GCL_OK) return s;
+    if ((s = check_max_aut(max_aut)) != GCL_OK) return s;
+    if (b->num_graphs < 1) return fail(GCL_E_INVALID, "empty batch");
+    BatchTuning bt{};
+    s = batch_autotune(*b, max_aut, kBatchRing, sample, &bt, best_ms, tried);
+    if (s != GCL_OK) return s;
+    best->spec_min_n = (int32_t)bt.spec_min_n;
+    best->min_class_n = (int32_t)bt.min_class_n;
+    return GCL_OK;
+}
+
 gcl_status gcl_device_batch_run(gcl_device_batch* db, int32_t use_graph, gcl_batch_stats* stats) {
     if (!db) return fail(GCL_E_INVALID, "null handle");
     auto t0 = std::chrono::steady_clock::now();

@@ -45,7 +45,21 @@ gcl_status gpu_canon_batch(const gcl_batch& b, uint32_t max_aut, uint32_t ring_c
 
 // Batch resident on the device (opaque); see gcl_device_batch_* in gcl.h.
 struct DeviceBatch;
-gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out);
+// Launch choices of a device batch: graphs with n >= spec_min_n publish their
+// root children (0: no speculation); graphs below min_class_n run in the
+// min_class_n size class (fewer, larger classes).
+struct BatchTuning {
+    uint32_t spec_min_n;
+    uint32_t min_class_n;
+};
+BatchTuning default_batch_tuning();
+gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out,
+                               const BatchTuning* tuning = nullptr);
+// Times every candidate tuning on an evenly strided sample of at most
+// `sample` graphs of b (one untimed run, then the best of two) and returns the
+// fastest; `tried` (optional) receives the number of candidates.
+gcl_status batch_autotune(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t sample, BatchTuning* best,
+                          double* best_ms, int32_t* tried);
 gcl_status device_batch_run(DeviceBatch* db, bool use_graph, float* ms);
 gcl_status device_batch_results(DeviceBatch* db, int32_t* out_label, DeviceGraphResult* res, gcl_batch_stats* st);
 gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32_t cap, int32_t* count);

@@ -176,6 +176,7 @@ struct DeviceBatch {
     int32_t workers = 0;
     bool timed_run = false;  // last run carried per-class events
     std::string side;        // side-file prefix taken at creation ("" = none)
+    BatchTuning tune{};
 
     ~DeviceBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -192,8 +193,9 @@ struct DeviceBatch {
         if (main) (void)hipStreamDestroy(main);
     }
 
-    gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap) {
+    gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap, const BatchTuning& t) {
         side = side_file_prefix();
+        tune = t;
         if (max_aut_ == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
         G = b.num_graphs;
         if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
@@ -216,7 +218,7 @@ struct DeviceBatch {
                 return fail(GCL_E_CAPACITY, "graph %d has n=%lld > %d", g, (long long)n, kMaxVertices);
             int cls = kClasses;
             for (int c = 0; c < kClasses; ++c)
-                if ((uint32_t)n <= kClassN[c]) {
+                if ((uint32_t)n <= kClassN[c] && kClassN[c] >= tune.min_class_n) {
                     cls = c;
                     break;
                 }
@@ -224,7 +226,7 @@ struct DeviceBatch {
             cls_maxn[cls] = std::max(cls_maxn[cls], (uint32_t)n);
             const uint64_t nnz = (uint64_t)(b.row_ptr[b.vtx_off[g + 1]] - b.row_ptr[b.vtx_off[g]]);
             cls_maxnnz[cls] = std::max(cls_maxnnz[cls], nnz);
-            if ((uint32_t)n >= spec_min_n()) {
+            if ((uint32_t)n >= tune.spec_min_n) {
                 cls_spec_work[cls] += (uint64_t)n;
                 cls_spec_big[cls] += 1;
             }
@@ -288,7 +290,7 @@ struct DeviceBatch {
             const bool hot_global = cls == kClasses;
             // per-level snapshots of the cell-boundary bitmap (depth <= n)
             const uint32_t levels = std::min<uint32_t>(maxn + 2, 4096u);
-            const uint32_t spec_min = spec_min_n();
+            const uint32_t spec_min = tune.spec_min_n;
             const bool spec = spec_min > 0 && cls_maxn[cls] >= spec_min;
             int per_cu = 0;
             GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls), kWave, 0));
@@ -882,10 +884,22 @@ void serve_stop(ServeEngine* s) { s->stop(); }
 gcl_status serve_wait(ServeEngine* s) { return s->wait(); }
 gcl_status serve_check(ServeEngine* s) { return s->check(); }
 
-gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out) {
+BatchTuning default_batch_tuning() {
+    BatchTuning t;
+    t.spec_min_n = spec_min_n();
+    t.min_class_n = kClassN[0];
+    return t;
+}
+
+gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out,
+                               const BatchTuning* tuning) {
+    const BatchTuning t = tuning ? *tuning : default_batch_tuning();
+    bool class_ok = false;
+    for (int c = 0; c < kClasses; ++c) class_ok |= t.min_class_n == kClassN[c];
+    if (!class_ok) return fail(GCL_E_INVALID, "min_class_n %u is not a size class bound", t.min_class_n);
     DeviceBatch* db = new (std::nothrow) DeviceBatch();
     if (!db) return fail(GCL_E_NOMEM, "out of host memory");
-    gcl_status s = db->create(b, max_aut, ring_cap);
+    gcl_status s = db->create(b, max_aut, ring_cap, t);
     if (s != GCL_OK) {
         delete db;
         return s;
@@ -907,6 +921,62 @@ gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32
 void device_batch_destroy(DeviceBatch* db) { delete db; }
 
 int32_t device_batch_size(const DeviceBatch* db) { return db->G; }
+
+gcl_status batch_autotune(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t sample, BatchTuning* best,
+                          double* best_ms, int32_t* tried) {
+    if (sample < 1) return fail(GCL_E_INVALID, "sample must be >= 1");
+    // evenly strided sample: same mix of sizes and kinds as the batch
+    const int32_t G = b.num_graphs;
+    const int32_t stride = std::max<int32_t>(1, (G + sample - 1) / sample);
+    std::vector<int64_t> vtx{0}, row{0};
+    std::vector<int32_t> col, color;
+    for (int32_t g = 0; g < G; g += stride) {
+        const int64_t v0 = b.vtx_off[g], v1 = b.vtx_off[g + 1];
+        const int64_t base = b.row_ptr[v0];
+        for (int64_t v = v0; v < v1; ++v) {
+            row.push_back(row.back() + (b.row_ptr[v + 1] - b.row_ptr[v]));
+            if (b.color) color.push_back(b.color[v]);
+        }
+        col.insert(col.end(), b.col + base, b.col + b.row_ptr[v1]);
+        vtx.push_back(vtx.back() + (v1 - v0));
+    }
+    gcl_batch s;
+    s.num_graphs = (int32_t)vtx.size() - 1;
+    s.vtx_off = vtx.data();
+    s.row_ptr = row.data();
+    s.col = col.empty() ? nullptr : col.data();
+    s.color = b.color ? color.data() : nullptr;
+    // candidates: speculation threshold x smallest size class
+    const uint32_t spec_c[] = {0u, 64u, 128u, 256u};
+    const uint32_t cls_c[] = {kClassN[0], kClassN[1], kClassN[2]};
+    double best_t = -1.0;
+    int32_t n_tried = 0;
+    for (uint32_t sc : spec_c)
+        for (uint32_t cc : cls_c) {
+            BatchTuning t;
+            t.spec_min_n = sc;
+            t.min_class_n = cc;
+            DeviceBatch* db = nullptr;
+            gcl_status st = device_batch_create(s, max_aut, ring_cap, &db, &t);
+            if (st != GCL_OK) return st;
+            float ms = 0.f;
+            double tb = 1e300;
+            for (int r = 0; r < 3 && st == GCL_OK; ++r) {
+                st = db->run(false, &ms);
+                if (r > 0) tb = std::min(tb, (double)ms);
+            }
+            delete db;
+            if (st != GCL_OK) return st;
+            ++n_tried;
+            if (best_t < 0 || tb < best_t) {
+                best_t = tb;
+                *best = t;
+            }
+        }
+    if (best_ms) *best_ms = best_t;
+    if (tried) *tried = n_tried;
+    return GCL_OK;
+}
 
 gcl_status gpu_canon_batch(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, int32_t* out_label,
                            DeviceGraphResult* results, gcl_batch_stats* st) {

@@ -112,6 +112,9 @@ _SIGS = {
     "gcl_device_count": (C.c_int32, []),
     "gcl_set_side_files": (C.c_int, [C.c_char_p]),
     "gcl_device_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gcl_batch_tuning_default": (C.c_int, [C.c_void_p]),
+    "gcl_device_batch_create_tuned": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "gcl_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_device_batch_destroy": (None, [C.c_void_p]),
@@ -425,14 +428,41 @@ def mix_batch(count, seed, n_min=16, n_max=1024, threads=0) -> Batch:
     return Batch.from_arrays(vtx, rows, col, None, kind)
 
 
-class DeviceBatch:
-    """A batch resident in device memory (gcl_device_batch_*)."""
+class _Tuning(C.Structure):
+    _fields_ = [("spec_min_n", C.c_int32), ("min_class_n", C.c_int32)]
 
-    def __init__(self, batch: Batch, max_aut=10000):
+
+def batch_tuning_default():
+    """The default launch tuning (gcl_batch_tuning_default) as a dict."""
+    t = _Tuning()
+    _check(lib().gcl_batch_tuning_default(C.byref(t)))
+    return {"spec_min_n": t.spec_min_n, "min_class_n": t.min_class_n}
+
+
+def autotune(batch: "Batch", max_aut=10000, sample=4096):
+    """gcl_batch_autotune: the fastest launch tuning on a strided sample of
+    the batch; returns (tuning dict, its kernel ms, candidates tried)."""
+    t = _Tuning()
+    ms = C.c_double(0.0)
+    tried = C.c_int32(0)
+    c = batch._c()
+    _check(lib().gcl_batch_autotune(C.byref(c), max_aut, sample, C.byref(t), C.byref(ms), C.byref(tried)))
+    return {"spec_min_n": t.spec_min_n, "min_class_n": t.min_class_n}, ms.value, tried.value
+
+
+class DeviceBatch:
+    """A batch resident in device memory (gcl_device_batch_*); `tuning` is a
+    dict with spec_min_n and min_class_n (gcl_device_batch_create_tuned)."""
+
+    def __init__(self, batch: Batch, max_aut=10000, tuning=None):
         self.batch = batch
         self._c = batch._c()
         self.h = C.c_void_p()
-        _check(lib().gcl_device_batch_create(C.byref(self._c), max_aut, C.byref(self.h)))
+        if tuning is None:
+            _check(lib().gcl_device_batch_create(C.byref(self._c), max_aut, C.byref(self.h)))
+        else:
+            t = _Tuning(int(tuning["spec_min_n"]), int(tuning["min_class_n"]))
+            _check(lib().gcl_device_batch_create_tuned(C.byref(self._c), max_aut, C.byref(t), C.byref(self.h)))
 
     def run(self, use_graph=False):
         st = _BatchStats()

@@ -211,6 +211,32 @@ gcl_status gcl_device_batch_class_stats(gcl_device_batch* db, gcl_class_stats* o
                                         int32_t* count);
 void gcl_device_batch_destroy(gcl_device_batch* db);
 
+/* Launch tuning of a device batch (no counterpart in the reference, whose one
+ * kernel shape is fixed at nauty_parallel.cu:640-760; this is the project's
+ * own autotuner over the batched launch):
+ *   spec_min_n   graphs with at least this many vertices publish their root
+ *                children for idle workers (0: no speculation); default 128,
+ *                or GCL_SPEC_MIN_N
+ *   min_class_n  smallest size class used (64, 128 or 256 ...): smaller
+ *                graphs run in it, so there are fewer, larger class kernels;
+ *                default 64
+ * gcl_device_batch_create_tuned is gcl_device_batch_create with a tuning
+ * (NULL: the defaults).  gcl_batch_autotune times every candidate (spec_min_n
+ * in {0, 64, 128, 256} x min_class_n in {64, 128, 256}) on an evenly strided
+ * sample of at most `sample` graphs of b: one untimed run, then the best of
+ * two.  It returns the fastest in *best, its kernel time in *best_ms and the
+ * number of candidates in *tried (both optional).  Results do not depend on
+ * the tuning: labels and node counts are the same under every candidate. */
+typedef struct gcl_batch_tuning {
+    int32_t spec_min_n;
+    int32_t min_class_n;
+} gcl_batch_tuning;
+gcl_status gcl_batch_tuning_default(gcl_batch_tuning* t);
+gcl_status gcl_device_batch_create_tuned(const gcl_batch* b, uint32_t max_aut, const gcl_batch_tuning* t,
+                                         gcl_device_batch** out);
+gcl_status gcl_batch_autotune(const gcl_batch* b, uint32_t max_aut, int32_t sample, gcl_batch_tuning* best,
+                              double* best_ms, int32_t* tried);
+
 /* Canonical keys of one graph from a labeling: out_keys[nnz] = sorted
  * i*n+j over directed edges of the relabeled graph. */
 gcl_status gcl_canonical_keys(const gcl_csr* g, const int32_t* label, uint32_t* out_keys);
