@@ -21,6 +21,12 @@ struct DiagRecord {
     uint64_t t_wait;   // within t_search: waiting for root children from helpers
     uint64_t t_help;   // refining root children for other workers
     uint64_t t_idle;   // backing off with nothing to do
+    // within t_search: individualize + refine, node classification (leaf
+    // certificates, automorphisms, the backtracking it decides), orbit
+    // updates, and backtracking past an exhausted target cell
+    uint64_t t_refine, t_classify, t_orbits, t_backtrack;
+    uint32_t adopted;   // root children taken refined from a helper
+    uint32_t rejected;  // root children rejected by the helper's code alone
 };
 
 // Per-graph result of a device launch.

@@ -461,17 +461,20 @@ struct DeviceBatch {
         FILE* f = fopen(fc.c_str(), "w");
         if (!f) return fail(GCL_E_IO, "cannot write %s", fc.c_str());
         fprintf(f, "WORKER,CLASS_MAX_N,XCC,SE,CU,GRAPHS,NODES,SPLITTER_STEPS,HELPED_CHILDREN,"
-                   "SEARCH_pct,WAIT_CHILDREN_pct,HELP_pct,IDLE_pct,TOTAL_TICKS\n");
+                   "SEARCH_pct,WAIT_CHILDREN_pct,HELP_pct,IDLE_pct,TOTAL_TICKS,"
+                   "REFINE_pct,CLASSIFY_pct,ORBITS_pct,BACKTRACK_pct,ADOPTED_CHILDREN,REJECTED_CHILDREN\n");
         std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint64_t> per_cu;
         for (size_t i = 0; i < rows.size(); ++i) {
             const DiagRecord& d = rows[i].second;
             const uint32_t xcc = d.hw_id >> 6, se = (d.hw_id >> 4) & 3u, cu = d.hw_id & 15u;
             const double tot = d.t_total ? (double)d.t_total : 1.0;
             const uint64_t own = d.t_search > d.t_wait ? d.t_search - d.t_wait : 0;
-            fprintf(f, "%zu,%d,%u,%u,%u,%u,%llu,%llu,%llu,%.2f,%.2f,%.2f,%.2f,%llu\n", i, rows[i].first, xcc, se, cu,
-                    d.graphs, (unsigned long long)d.nodes, (unsigned long long)d.steps,
+            fprintf(f, "%zu,%d,%u,%u,%u,%u,%llu,%llu,%llu,%.2f,%.2f,%.2f,%.2f,%llu,%.2f,%.2f,%.2f,%.2f,%u,%u\n", i,
+                    rows[i].first, xcc, se, cu, d.graphs, (unsigned long long)d.nodes, (unsigned long long)d.steps,
                     (unsigned long long)d.helped, 100.0 * (double)own / tot, 100.0 * (double)d.t_wait / tot,
-                    100.0 * (double)d.t_help / tot, 100.0 * (double)d.t_idle / tot, (unsigned long long)d.t_total);
+                    100.0 * (double)d.t_help / tot, 100.0 * (double)d.t_idle / tot, (unsigned long long)d.t_total,
+                    100.0 * (double)d.t_refine / tot, 100.0 * (double)d.t_classify / tot,
+                    100.0 * (double)d.t_orbits / tot, 100.0 * (double)d.t_backtrack / tot, d.adopted, d.rejected);
             per_cu[std::make_tuple(xcc, se, cu)] += d.nodes;
         }
         fclose(f);

@@ -481,6 +481,16 @@ struct Worker {
     bool root_changed;       // the partition held at level 1 is not the root's (own helping)
     // diagnostics: totals of this worker over the launch (DiagRecord)
     uint64_t dg_nodes, dg_steps, dg_helped, dg_search, dg_wait, dg_help, dg_idle;
+    // phase times of the own search (refine, classify, orbits, backtracking
+    // past an exhausted cell) and root children adopted from helpers or
+    // rejected by their code; kept only with diagnostics on
+    uint64_t dg_refine, dg_classify, dg_orbits, dg_backtrack;
+    uint32_t dg_adopted, dg_rejected;
+    bool dg_on;
+    __device__ __forceinline__ uint64_t dg_t0() const { return dg_on ? wall_clock64() : 0ull; }
+    __device__ __forceinline__ void dg_add(uint64_t& acc, uint64_t t0) {
+        if (dg_on) acc += wall_clock64() - t0;
+    }
     uint32_t dg_graphs;
     int32_t spec_slot;       // board slot published by this worker, or -1
     bool spec_pending;       // counted in *pending until it publishes or ends
@@ -1850,7 +1860,9 @@ struct Worker {
         seq[seq_size] = (ix_t)v;
         ++seq_size;
         group_sync();
+        const uint64_t t0 = dg_t0();
         individualize_and_refine(v, tc_first);
+        dg_add(dg_refine, t0);
         ++level;
         snapshot(level);
     }
@@ -2157,11 +2169,15 @@ struct Worker {
             const uint32_t li = level - 1;
             LevelRec r = lv[li];
             if (r.cursor >= r.tc_len) {
+                const uint64_t t0 = dg_t0();
                 backtrack(level - 1);
+                dg_add(dg_backtrack, t0);
                 return true;
             }
             if (num_aut > 0 && num_aut > r.last_num_aut) {
+                const uint64_t t0 = dg_t0();
                 compute_orbits(li);
+                dg_add(dg_orbits, t0);
                 lv[li].last_num_aut = num_aut;
             }
             // next member that is its own orbit representative
@@ -2188,11 +2204,13 @@ struct Worker {
             lv[li].cursor = cur + 1;
             const int taken = level == 1 ? spec_take(P, v) : kNotTaken;
             if (taken == kRejected) {
+                ++dg_rejected;
                 // classified and backtracked already (spec_take)
                 ++nodes;
                 return true;
             }
             if (taken == kTaken) {
+                ++dg_adopted;
                 ++nodes;
                 seq[seq_size] = (ix_t)v;
                 ++seq_size;
@@ -2203,7 +2221,9 @@ struct Worker {
                 descend(v, r.tc_first);
             }
         }
+        const uint64_t t0 = dg_t0();
         classify();
+        dg_add(dg_classify, t0);
         return true;
     }
 
@@ -2411,6 +2431,9 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.deadline = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
     W.steps = 0;
     W.dg_nodes = W.dg_steps = W.dg_helped = W.dg_search = W.dg_wait = W.dg_help = W.dg_idle = 0;
+    W.dg_refine = W.dg_classify = W.dg_orbits = W.dg_backtrack = 0;
+    W.dg_adopted = W.dg_rejected = 0;
+    W.dg_on = P.diag != nullptr;
     W.dg_graphs = 0;
 }
 
@@ -2429,6 +2452,12 @@ __device__ void write_diag(Worker<Hot>& W, const KParams& P, uint64_t t_start) {
     r.t_wait = W.dg_wait;
     r.t_help = W.dg_help;
     r.t_idle = W.dg_idle;
+    r.t_refine = W.dg_refine;
+    r.t_classify = W.dg_classify;
+    r.t_orbits = W.dg_orbits;
+    r.t_backtrack = W.dg_backtrack;
+    r.adopted = W.dg_adopted;
+    r.rejected = W.dg_rejected;
     P.diag[blockIdx.x] = r;
 }
 

@@ -41,10 +41,17 @@ def test_side_files_of_a_batch(tmp_path):
     assert sum(int(x["GRAPHS"]) for x in rows) == len(graphs)
     assert sum(int(x["NODES"]) for x in rows) == r.total_nodes
     assert sum(int(x["HELPED_CHILDREN"]) for x in rows) > 0  # the regular graphs publish root children
+    # root children published by the regular graphs are adopted from helpers
+    # or rejected by their code
+    assert sum(int(x["ADOPTED_CHILDREN"]) + int(x["REJECTED_CHILDREN"]) for x in rows) > 0
     for x in rows:
         assert int(x["XCC"]) < 16 and int(x["SE"]) < 4 and int(x["CU"]) < 16
         pct = [float(x[k]) for k in ("SEARCH_pct", "WAIT_CHILDREN_pct", "HELP_pct", "IDLE_pct")]
         assert all(0.0 <= p <= 100.0 for p in pct) and sum(pct) <= 100.5
+        # the phases of the own search lie within its share
+        phases = [float(x[k]) for k in ("REFINE_pct", "CLASSIFY_pct", "ORBITS_pct", "BACKTRACK_pct")]
+        assert all(p >= 0.0 for p in phases) and sum(phases) <= float(x["SEARCH_pct"]) + 0.5
+    assert sum(float(x["REFINE_pct"]) for x in rows if int(x["GRAPHS"]) > 0) > 0.0
     cu = _rows(prefix + "_nodes_per_cu.csv")
     assert sum(int(x["nodes"]) for x in cu) == r.total_nodes
     # off again: a later launch writes nothing
