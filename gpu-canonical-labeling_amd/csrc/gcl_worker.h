@@ -1246,38 +1246,20 @@ struct Worker {
         // hash chain / active rules in position order (reference 253-468)
         if (s > 1) {
             hash_pass_parallel(X, nns, longcode, hint);
-        } else
-        for (uint32_t base = 0; base < nns; base += kWave) {
-            const uint32_t i = base + (uint32_t)l;
-            // cells that did not split hash their common count (0 when
-            // untouched) plus their first position
-            uint32_t f = 0, x = 0;
-            bool sp = false;
-            if (i < nns) {
-                f = H.sb(i);
-                sp = test_bit(H.split(), f);
-                x = (s > 1 && !sp) ? H.cnt_ld(H.elem(f)) + f : f;
-            }
-            const uint64_t special = ballot(sp);
-            const uint32_t here = min((uint32_t)kWave, nns - base);
-            for (uint32_t t = 0; t < here; ++t) {
-                // run of cells that did not split, up to the next split one
-                const uint64_t rest = special >> t;
-                const uint32_t nxt = rest ? t + (uint32_t)__builtin_ctzll(rest) : here;
-                if (s > 1) {
-                    for (; t + 4 <= nxt; t += 4) {
-                        longcode = GCL_MASH(longcode, readlane(x, (int)t));
-                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 1));
-                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 2));
-                        longcode = GCL_MASH(longcode, readlane(x, (int)t + 3));
-                    }
-                    for (; t < nxt; ++t) longcode = GCL_MASH(longcode, readlane(x, (int)t));
+        } else {
+            // single-vertex splitter: cells that did not split add nothing to
+            // the chain; each split cell, in position order, split in two
+            // (neighbours first)
+            for (uint32_t base = 0; base < nns; base += kWave) {
+                const uint32_t i = base + (uint32_t)l;
+                uint32_t f = 0;
+                bool sp = false;
+                if (i < nns) {
+                    f = H.sb(i);
+                    sp = test_bit(H.split(), f);
                 }
-                t = nxt;
-                if (t >= here) break;
-                const uint32_t fc = readlane(f, (int)t);
-                if (s == 1) {
-                    // two fragments: neighbours first
+                for (uint64_t m = ballot(sp); m; m &= m - 1) {
+                    const uint32_t fc = readlane(f, __builtin_ctzll(m));
                     const uint32_t K = uni((uint32_t)H.cend(fc)) - fc + 1;
                     const uint32_t LL = uni((uint32_t)H.cend(fc + K)) - fc + 1;
                     longcode = GCL_MASH(longcode, fc + K - 1);
@@ -1288,32 +1270,6 @@ struct Worker {
                         add_active(X, fc);
                         if (K == 1) hint = (int)fc;
                     }
-                    continue;
-                }
-                // fragments of the old cell, up to the one ending at its old last
-                uint32_t max_size = 0, max_first = fc;
-                for (uint32_t q = fc;;) {
-                    const uint32_t last = uni((uint32_t)H.cend(q));
-                    const uint32_t size = last - q + 1;
-                    longcode = GCL_MASH(longcode, uni(H.cnt_ld(H.elem(q))) + q);
-                    if (size > max_size) {
-                        max_size = size;
-                        max_first = q;
-                    }
-                    if (uni(test_bit(H.oldend(), last)) || last + 1 >= X.n) break;
-                    q = last + 1;
-                }
-                const bool was_active = uni(test_bit(H.active(), fc));
-                for (uint32_t q = uni((uint32_t)H.cend(fc)) + 1;;) {
-                    const uint32_t last = uni((uint32_t)H.cend(q));
-                    add_active(X, q);
-                    if (last == q) hint = (int)q;
-                    if (uni(test_bit(H.oldend(), last)) || last + 1 >= X.n) break;
-                    q = last + 1;
-                }
-                if (!was_active) {
-                    add_active(X, fc);
-                    remove_active(X, max_first);
                 }
             }
         }
