@@ -445,20 +445,40 @@ void CpuSearch::compute_orbits(LevelRec& r) {
     if (subgroup_.empty()) return;
     uint32_t* mem = &arena_mem_[r.arena_off];
     uint32_t* rep = &arena_rep_[r.arena_off];
-    for (uint32_t v = 0; v < n_; ++v) dsu_[v] = v;
-    for (uint32_t i = 0; i < r.tc_len; ++i) dsu_[mem[i]] = rep[i];
     auto find = [&](uint32_t x) {
         while (dsu_[x] != x) x = dsu_[x];
         return x;
     };
+    auto unite = [&](uint32_t v, uint32_t w) {
+        const uint32_t x = find(v), y = find(w);
+        if (x < y) dsu_[y] = x;
+        else if (y < x) dsu_[x] = y;
+    };
+    // An automorphism that fixes the sequence maps this node's partition to
+    // itself (refinement is label-invariant), so it permutes the target
+    // cell: unions over the cell's members give the members' orbits, whose
+    // minima are the representatives.  An image outside the cell would mean
+    // otherwise; then every vertex takes part, as in the reference.
+    bool inside = true;
     for (int64_t s : subgroup_) {
         const uint32_t* a = &auts_[(size_t)s * n_];
-        for (uint32_t v = 0; v < n_; ++v) {
-            if (a[v] == v) continue;
-            uint32_t x = find(v), y = find(a[v]);
-            if (x == y) continue;
-            if (x < y) dsu_[y] = x;
-            else dsu_[x] = y;
+        for (uint32_t i = 0; i < r.tc_len && inside; ++i)
+            inside = pos_[a[mem[i]]] - r.tc_first < r.tc_len;
+    }
+    if (inside) {
+        for (uint32_t i = 0; i < r.tc_len; ++i) dsu_[mem[i]] = rep[i];
+        for (int64_t s : subgroup_) {
+            const uint32_t* a = &auts_[(size_t)s * n_];
+            for (uint32_t i = 0; i < r.tc_len; ++i)
+                if (a[mem[i]] != mem[i]) unite(mem[i], a[mem[i]]);
+        }
+    } else {
+        for (uint32_t v = 0; v < n_; ++v) dsu_[v] = v;
+        for (uint32_t i = 0; i < r.tc_len; ++i) dsu_[mem[i]] = rep[i];
+        for (int64_t s : subgroup_) {
+            const uint32_t* a = &auts_[(size_t)s * n_];
+            for (uint32_t v = 0; v < n_; ++v)
+                if (a[v] != v) unite(v, a[v]);
         }
     }
     for (uint32_t i = 0; i < r.tc_len; ++i) rep[i] = find(mem[i]);
