@@ -68,3 +68,21 @@ def test_server_config_defaults_and_no_device():
             raise AssertionError("expected GclError")
         except gcl.GclError as e:
             assert e.status == -4  # no device
+
+
+def test_batch_tuning_arguments_rejected_without_device(monkeypatch):
+    monkeypatch.delenv("GCL_SPEC_MIN_N", raising=False)
+    assert gcl.batch_tuning_default() == {"spec_min_n": 128, "min_class_n": 64}
+    batch = gcl.Batch([gcl.torus(3, 3), gcl.torus(4, 4)])
+    bad = ({"spec_min_n": -1, "min_class_n": 64}, {"spec_min_n": 128, "min_class_n": 100})
+    for t in bad:
+        try:
+            gcl.DeviceBatch(batch, tuning=t)
+            raise AssertionError("expected GclError")
+        except gcl.GclError as e:
+            assert e.status == -1, t
+    try:
+        gcl.autotune(batch, sample=0)
+        raise AssertionError("expected GclError")
+    except gcl.GclError as e:
+        assert e.status == -1
