@@ -142,6 +142,40 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
     return GCL_OK;
 }
 
+gcl_status gcl_canon_automorphisms(const gcl_csr* g, uint32_t max_aut, int32_t* out_label, int32_t* out_auts,
+                                   int64_t cap_rows, int64_t* rows, gcl_stats* stats) {
+    gcl_status s = validate_nonempty(g);
+    if (s != GCL_OK) return s;
+    if ((s = check_max_aut(max_aut)) != GCL_OK) return s;
+    if (cap_rows < 0 || (cap_rows > 0 && !out_auts)) return fail(GCL_E_INVALID, "bad automorphism buffer");
+    OneBatch ob(g);
+    BatchTuning t = default_batch_tuning();
+    t.keep_auts = 1;
+    DeviceBatch* db = nullptr;
+    // the single-graph ring: max_aut slots, as gcl_canon_label uses
+    s = device_batch_create(ob.b, max_aut, max_aut, &db, &t);
+    if (s != GCL_OK) return s;
+    float ms = 0.f;
+    std::vector<DeviceGraphResult> res(1);
+    gcl_batch_stats bs;
+    memset(&bs, 0, sizeof(bs));
+    s = device_batch_run(db, false, &ms);
+    if (s == GCL_OK) s = device_batch_results(db, out_label, res.data(), &bs);
+    if (s == GCL_OK) s = device_batch_automorphisms(db, 0, out_auts, cap_rows, rows);
+    device_batch_destroy(db);
+    if (s != GCL_OK) return s;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->nodes = res[0].nodes;
+        stats->automorphisms = res[0].automorphisms;
+        stats->refinements = res[0].nodes;
+        stats->splitter_steps = res[0].splitter_steps;
+        stats->csr_bytes = res[0].csr_bytes;
+        stats->kernel_ms = ms;
+    }
+    return GCL_OK;
+}
+
 gcl_status gcl_set_side_files(const char* prefix) {
     gcl::set_side_file_prefix(prefix);
     return GCL_OK;

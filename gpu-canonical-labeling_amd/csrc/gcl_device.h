@@ -57,8 +57,13 @@ struct DeviceBatch;
 struct BatchTuning {
     uint32_t spec_min_n;
     uint32_t min_class_n;
+    uint32_t keep_auts = 0;  // one automorphism ring per graph, readable after the run
 };
 BatchTuning default_batch_tuning();
+// Automorphisms stored for graph g in the last run of a batch created with
+// keep_auts: min(found, ring) rows of n images (row-major, a[v] per vertex v)
+// into out (cap_rows rows at most); *rows receives the rows stored.
+gcl_status device_batch_automorphisms(DeviceBatch* db, int32_t g, int32_t* out, int64_t cap_rows, int64_t* rows);
 gcl_status device_batch_create(const gcl_batch& b, uint32_t max_aut, uint32_t ring_cap, DeviceBatch** out,
                                const BatchTuning* tuning = nullptr);
 // Times every candidate tuning on an evenly strided sample of at most

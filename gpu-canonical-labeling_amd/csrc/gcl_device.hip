@@ -163,6 +163,7 @@ struct DeviceBatch {
         hipEvent_t done = nullptr;
         hipEvent_t t0 = nullptr, t1 = nullptr;  // timing of the class kernel on its stream
         std::vector<int32_t> graphs;            // host copy of the class's items
+        uint64_t aut_stride = 0;                // bytes per graph's ring in autpool (0: not pooled)
     };
     int32_t G = 0;
     int64_t total_n = 0, total_nnz = 0;
@@ -177,6 +178,7 @@ struct DeviceBatch {
     bool timed_run = false;  // last run carried per-class events
     std::string side;        // side-file prefix taken at creation ("" = none)
     BatchTuning tune{};
+    std::vector<int64_t> vtx_h;  // host copy of the graph offsets
 
     ~DeviceBatch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -196,6 +198,7 @@ struct DeviceBatch {
     gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap, const BatchTuning& t) {
         side = side_file_prefix();
         tune = t;
+        if (b.num_graphs >= 0) vtx_h.assign(b.vtx_off, b.vtx_off + b.num_graphs + 1);
         if (max_aut_ == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
         G = b.num_graphs;
         if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
@@ -301,7 +304,7 @@ struct DeviceBatch {
             // one ring per graph in a pool (large rings: the reference-shaped
             // entry points use max_aut rows of n)
             const uint64_t ring_bytes = aut_region_bytes(ring, maxn);
-            const bool pool_rings = ring_bytes > (1ull << 20) && (int64_t)cnt < resident;
+            const bool pool_rings = tune.keep_auts || (ring_bytes > (1ull << 20) && (int64_t)cnt < resident);
             const Layout L = make_layout(maxn, max_nnz, ring, arena, levels, hot_global, pool_rings);
             size_t free_b = 0, total_b = 0;
             GCL_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -377,6 +380,7 @@ struct DeviceBatch {
             P.diag = side.empty() ? nullptr : (DiagRecord*)C.diag.p;
             P.aut_pool = pool_rings ? (uint8_t*)C.autpool.p : nullptr;
             P.aut_stride = pool_rings ? ring_bytes : 0;
+            C.aut_stride = P.aut_stride;
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));
@@ -515,6 +519,31 @@ struct DeviceBatch {
             }
         }
         return GCL_OK;
+    }
+
+    gcl_status automorphisms(int32_t g, int32_t* dst, int64_t cap_rows, int64_t* rows) {
+        if (g < 0 || g >= G) return fail(GCL_E_INVALID, "graph %d out of range", g);
+        GCL_HIP(hipSetDevice(0));
+        DeviceGraphResult r;
+        GCL_HIP(hipMemcpy(&r, (DeviceGraphResult*)out.p + g, sizeof(r), hipMemcpyDeviceToHost));
+        for (auto& c : classes) {
+            for (size_t item = 0; item < c.graphs.size(); ++item) {
+                if (c.graphs[item] != g) continue;
+                if (c.aut_stride == 0) return fail(GCL_E_INVALID, "batch was not created with keep_auts");
+                const int64_t n = vtx_h[(size_t)g + 1] - vtx_h[(size_t)g];
+                const int64_t stored = std::min<int64_t>((int64_t)r.automorphisms, (int64_t)ring);
+                const int64_t k = std::min<int64_t>(stored, cap_rows);
+                if (k > 0 && n > 0) {
+                    std::vector<ix_t> buf((size_t)(k * n));
+                    const uint8_t* base = (const uint8_t*)c.autpool.p + item * c.aut_stride + aut_fix_bytes(ring);
+                    GCL_HIP(hipMemcpy(buf.data(), base, sizeof(ix_t) * buf.size(), hipMemcpyDeviceToHost));
+                    for (size_t i = 0; i < buf.size(); ++i) dst[i] = (int32_t)buf[i];
+                }
+                if (rows) *rows = stored;
+                return GCL_OK;
+            }
+        }
+        return fail(GCL_E_INTERNAL, "graph %d is in no class", g);
     }
 
     gcl_status results(int32_t* out_label, DeviceGraphResult* res_out, gcl_batch_stats* st) {
@@ -922,6 +951,10 @@ gcl_status device_batch_class_stats(DeviceBatch* db, gcl_class_stats* out, int32
 }
 
 void device_batch_destroy(DeviceBatch* db) { delete db; }
+
+gcl_status device_batch_automorphisms(DeviceBatch* db, int32_t g, int32_t* out, int64_t cap_rows, int64_t* rows) {
+    return db->automorphisms(g, out, cap_rows, rows);
+}
 
 int32_t device_batch_size(const DeviceBatch* db) { return db->G; }
 

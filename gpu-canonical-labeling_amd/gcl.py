@@ -113,6 +113,8 @@ _SIGS = {
     "gcl_set_side_files": (C.c_int, [C.c_char_p]),
     "gcl_device_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "gcl_batch_tuning_default": (C.c_int, [C.c_void_p]),
+    "gcl_canon_automorphisms": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                          C.c_void_p]),
     "gcl_device_batch_create_tuned": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "gcl_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_device_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
@@ -430,6 +432,20 @@ def mix_batch(count, seed, n_min=16, n_max=1024, threads=0) -> Batch:
 
 class _Tuning(C.Structure):
     _fields_ = [("spec_min_n", C.c_int32), ("min_class_n", C.c_int32)]
+
+
+def canon_automorphisms(g: "Graph", max_aut=10000):
+    """gcl_canon_automorphisms: (labels, automorphisms as an array of rows
+    a with a[v] the image of v, automorphisms found) from the GPU search."""
+    c = g._csr()
+    label = np.zeros(max(g.n, 1), np.int32)
+    cap = max_aut
+    auts = np.zeros((cap, max(g.n, 1)), np.int32)
+    rows = C.c_int64(0)
+    st = _Stats()
+    _check(lib().gcl_canon_automorphisms(C.byref(c), max_aut, _ptr(label), _ptr(auts), cap, C.byref(rows),
+                                         C.byref(st)))
+    return label[:g.n], auts[:rows.value, :g.n], st.automorphisms
 
 
 def batch_tuning_default():

@@ -118,6 +118,18 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
                            int32_t* out_label, uint32_t* out_keys,
                            uint8_t* out_dense, gcl_stats* stats);
 
+/* The automorphisms the GPU search stored for one graph, for callers that
+ * check or reuse them.  No counterpart in the reference: its store is
+ * internal (sequential.cpp:515,552; nauty_parallel.cu).  Labels as
+ * gcl_canon_label (out_label may be NULL); out_auts receives up to cap_rows
+ * rows of g->n ints, row r holding the image a(v) of every vertex v; *rows
+ * (optional) the number stored, min(automorphisms found, max_aut) (beyond
+ * max_aut the ring keeps the latest).  Each is a leaf-to-leaf map whose
+ * certificates compared equal, hence an automorphism of g preserving
+ * colours. */
+gcl_status gcl_canon_automorphisms(const gcl_csr* g, uint32_t max_aut, int32_t* out_label, int32_t* out_auts,
+                                   int64_t cap_rows, int64_t* rows, gcl_stats* stats);
+
 /*
  * Isomorphism verdict of the two-graph CLI mode (main.cpp:173-198):
  * *verdict = 1 when the canonical forms agree, 0 otherwise.  Graphs with
