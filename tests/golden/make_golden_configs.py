@@ -2,11 +2,14 @@
 (oracle/_ref/gpu_canon_label_ref) on graphs of the benchmark configurations'
 full sizes (3-regular and torus graphs with n = 4096, CFI graphs with
 n = 1000, 2000 and 4000, each with a relabeled copy and a double-edge-swap
-variant where the configuration names one).  The outputs are large, so the
+variant where the configuration names one), and two of the serving
+configuration's large graphs (G(n, p) of mean degree 8 with n = 20,000 and a
+3-regular graph with n = 10,000).  The outputs are large, so the
 fixture keeps their SHA-256 and length with the node and automorphism
 counts; the inputs are re-created from libgcl's seeded generators, pinned by
 the SHA-256 of their DIMACS text.  Takes about 10 minutes of one CPU core
-(the reference needs ~150 s per 3-regular graph).
+for the n <= 4096 graphs (the reference needs ~150 s per 3-regular graph)
+and most of an hour for the 3-regular n = 10,000 graph.
 
     python tests/golden/make_golden_configs.py [NAME ...]   (needs the reference sources)
 
@@ -43,6 +46,9 @@ RECIPES = {
     "cfi200_s2_twisted": ["cfi", 200, 1, 2],
     "cfi400_s3_straight": ["cfi", 400, 0, 3],
     "cfi400_s3_twisted": ["cfi", 400, 1, 3],
+    # the serving configuration's large graphs (searched by the global class)
+    "er_n20000_d8_s5": ["er_mean_degree", 20000, 8.0, 5],
+    "rr3_n10000_s7": ["random_regular", 10000, 3, 7],
 }
 
 
@@ -53,6 +59,8 @@ def build_graph(name, cache=None):
     r = RECIPES[name]
     if r[0] == "random_regular":
         g = gcl.random_regular(r[1], r[2], r[3])
+    elif r[0] == "er_mean_degree":
+        g = gcl.erdos_renyi_mean_degree(r[1], r[2], r[3])
     elif r[0] == "torus":
         g = gcl.torus(r[1], r[2])
     elif r[0] == "cfi":
@@ -88,7 +96,7 @@ def main(names):
             gcl.write_dimacs(f, g)
             dimacs_sha = hashlib.sha256(open(f, "rb").read()).hexdigest()
             p = subprocess.run([REF, f, "--sequential", "--max-aut=10000"], cwd=tmp, capture_output=True,
-                               text=True, timeout=3600)
+                               text=True, timeout=7200)
             if p.returncode != 0:
                 sys.exit(f"reference failed on {name}: {p.stderr}")
             data = open(os.path.join(tmp, "output"), "rb").read()

@@ -1,6 +1,8 @@
 """Reference outputs at the benchmark configurations' full sizes
 (tests/golden/configs.json, written by make_golden_configs.py from the
-reference's sequential binary): the GPU engine's `output` bytes (SHA-256),
+reference's sequential binary), the serving configuration's large graphs
+(G(n, p) of mean degree 8 with n = 20,000, searched by the global class, and
+a 3-regular graph with n = 10,000) included: the GPU engine's `output` bytes (SHA-256),
 node and automorphism counts equal the reference's; pairs built from them
 (relabeled copy, double-edge swap) get the reference's verdict.  The CPU part
 only checks that the seeded generators still produce the same inputs."""
@@ -33,6 +35,8 @@ def build(entries):
         r = by[name]["recipe"]
         if r[0] == "random_regular":
             g = gcl.random_regular(r[1], r[2], r[3])
+        elif r[0] == "er_mean_degree":
+            g = gcl.erdos_renyi_mean_degree(r[1], r[2], r[3])
         elif r[0] == "torus":
             g = gcl.torus(r[1], r[2])
         elif r[0] == "cfi":
