@@ -95,6 +95,7 @@ gcl_status gpu_label_one(const gcl_csr* g, uint32_t max_aut, int32_t* label, gcl
         st->refinements = res.nodes;
         st->splitter_steps = res.splitter_steps;
         st->csr_bytes = res.csr_bytes;
+        st->alg_bytes = res.csr_bytes + res.part_bytes;
         st->kernel_ms = bs.kernel_ms;
         st->total_ms = bs.total_ms;
     }
@@ -171,6 +172,7 @@ gcl_status gcl_canon_automorphisms(const gcl_csr* g, uint32_t max_aut, int32_t* 
         stats->refinements = res[0].nodes;
         stats->splitter_steps = res[0].splitter_steps;
         stats->csr_bytes = res[0].csr_bytes;
+        stats->alg_bytes = res[0].csr_bytes + res[0].part_bytes;
         stats->kernel_ms = ms;
     }
     return GCL_OK;
@@ -281,6 +283,7 @@ gcl_status gcl_batch_canon(const gcl_batch* b, uint32_t max_aut, gcl_engine engi
         acc.automorphisms += st.automorphisms;
         acc.splitter_steps += st.splitter_steps;
         acc.csr_bytes += st.csr_bytes;
+        acc.alg_bytes += st.alg_bytes;
         if (out_nodes) out_nodes[g] = st.nodes;
     }
     acc.workers = 1;
@@ -373,6 +376,7 @@ gcl_status gcl_device_batch_results(gcl_device_batch* db, int32_t* out_label, ui
         stats->automorphisms = bs.automorphisms;
         stats->splitter_steps = bs.splitter_steps;
         stats->csr_bytes = bs.csr_bytes;
+        stats->alg_bytes = bs.alg_bytes;
         stats->workers = bs.workers;
     }
     return s;
@@ -435,6 +439,7 @@ gcl_status gcl_batch_canon_plain(const gcl_batch* b, uint32_t max_aut, int32_t* 
         acc.automorphisms += res.automorphisms;
         acc.splitter_steps += res.splitter_steps;
         acc.csr_bytes += res.csr_bytes;
+        acc.alg_bytes += res.csr_bytes + res.part_bytes;
         acc.kernel_ms += bs.kernel_ms;
         acc.workers = std::max(acc.workers, bs.workers);
         if (out_nodes) out_nodes[g] = res.nodes;

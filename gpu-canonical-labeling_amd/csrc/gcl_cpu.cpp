@@ -111,6 +111,8 @@ class CpuSearch {
     std::vector<uint32_t> dsu_;
     std::vector<int64_t> subgroup_;
     uint64_t nodes_ = 0, refinements_ = 0, steps_ = 0, csr_bytes_ = 0;
+    // algorithmic bytes beyond the CSR reads (as the device counts them)
+    uint64_t part_bytes_ = 0;
 };
 
 void CpuSearch::init_partition() {
@@ -200,6 +202,7 @@ void CpuSearch::singleton_step(uint32_t w, uint32_t tag, uint32_t& longcode, int
         undo_tag_.push_back(tag);
         ++ncells_;
         splits_.push_back({f, k, L});
+        part_bytes_ += 8ull * L;
     }
     for (int32_t e = rp[w]; e < rp[w + 1]; ++e) mark_[(uint32_t)col[e]] = 0;
     std::sort(splits_.begin(), splits_.end(), [](const Split& a, const Split& b) { return a.f < b.f; });
@@ -259,6 +262,7 @@ void CpuSearch::multi_step(uint32_t p, uint32_t e, uint32_t tag, uint32_t& longc
         }
         cstate_[f] = 2;
         split_firsts_.push_back(f);
+        part_bytes_ += 8ull * L;
         sortbuf_.clear();
         for (uint32_t q = f; q <= c; ++q) sortbuf_.push_back({cnt_[elem_[q]], elem_[q]});
         std::sort(sortbuf_.begin(), sortbuf_.end());
@@ -293,6 +297,7 @@ void CpuSearch::multi_step(uint32_t p, uint32_t e, uint32_t tag, uint32_t& longc
         while (bits) {
             const uint32_t f = wi * 64 + (uint32_t)__builtin_ctzll(bits);
             bits &= bits - 1;
+            part_bytes_ += 8;  // the cell header hashed
             const uint8_t st = cstate_[f];
             cstate_[f] = 0;
             if (st == 0) {
@@ -358,6 +363,7 @@ int CpuSearch::refine(int32_t init_active, uint32_t tag) {
     }
     longcode = GCL_MASH(longcode, ncells_);
     ++refinements_;
+    part_bytes_ += 8ull * n_;
     return GCL_CLEANUP(longcode);
 }
 
@@ -726,6 +732,7 @@ gcl_status CpuSearch::run(int32_t* out_label, uint32_t* out_keys, gcl_stats* st)
         st->refinements = refinements_;
         st->splitter_steps = steps_;
         st->csr_bytes = csr_bytes_;
+        st->alg_bytes = csr_bytes_ + part_bytes_;
         st->kernel_ms = 0.0;
         st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -36,6 +36,7 @@ struct DeviceGraphResult {
     uint64_t splitter_steps;
     uint64_t csr_bytes;
     uint64_t device_ns;  // wall time of the graph's search on the device
+    uint64_t part_bytes; // algorithmic bytes of refinement beyond csr_bytes
     int32_t status;      // 0 ok, otherwise a gcl_status
     int32_t pad;
 };
@@ -85,7 +86,16 @@ struct ServeConfig {
     uint32_t ring_cap = 64;           // automorphism slots per worker
     int32_t slots = 0;                // request slots per class (0: twice the workers)
     uint64_t lifetime_ticks = ~0ull;  // device wall-clock ticks a worker serves at most
+    // share of the chip's worker slots for the class above 1024 vertices
+    // (when max_n > 1024); the first class gets 0.9 minus this
+    double large_share = 0.4;
     uint64_t scratch_bytes_per_class = 2ull << 30;
+    // the large class's workers hold snapshots and arenas sized by max_n
+    // (about 55 MB each at 50,000 vertices)
+    uint64_t scratch_bytes_large = 32ull << 30;
+    // the large class keeps up to 32 records (large graphs publishing at
+    // once), each holding every root child of a 10,000-vertex graph
+    uint64_t spec_bytes_large = 16ull << 30;
     // speculation records: one per worker, each large enough for every root
     // child of a class-sized graph (a record that holds fewer leaves the rest
     // to the owning worker alone: n = 1024 needs ~4.3 GiB over 1,843 records)
@@ -124,7 +134,9 @@ void serve_publish(ServeEngine* s, int k, uint32_t slot);  // slot data written 
 int serve_completions(ServeEngine* s, int k, uint32_t* out, int cap);
 void serve_stop(ServeEngine* s);
 gcl_status serve_wait(ServeEngine* s);
-gcl_status serve_check(ServeEngine* s);  // GCL_E_DEVICE once a serving kernel has failed
+// GCL_E_DEVICE once a serving kernel has failed, GCL_E_TIMEOUT once one has
+// returned because its lifetime passed
+gcl_status serve_check(ServeEngine* s);
 
 int32_t device_count();
 uint64_t device_wall_ticks_per_ms();
@@ -136,6 +148,15 @@ std::string side_file_prefix();
 // Smallest graph whose root children are published to helpers
 // (GCL_SPEC_MIN_N overrides; 0 disables).
 uint32_t spec_min_n(uint32_t dflt = 128);
-uint64_t time_limit_ticks();
+// Wall-clock limits in device ticks: a launch (or a serving worker) runs at
+// most `launch` ticks, a search at most base + per_n2 * n^2.
+struct TimeLimits {
+    uint64_t launch = ~0ull, base = ~0ull, per_n2 = 0;
+};
+// GCL_TIME_LIMIT_MS, when set, bounds the launch and each search by that
+// many milliseconds (0: no limit).  Otherwise a search may take
+// 60 s + n^2 / 10^4 ms and a launch 60 s + the sum of its graphs' n^2 /
+// 10^4 ms (sum_n2).
+TimeLimits time_limits(double sum_n2);
 
 }  // namespace gcl

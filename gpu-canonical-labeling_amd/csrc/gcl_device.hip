@@ -47,12 +47,12 @@ constexpr int kClasses = 8;
 constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
 
 template <int MAXN>
-hipError_t launch_lds(const KParams& P, int grid, hipStream_t s) {
+hipError_t launch_lds(const KParams* P, int grid, hipStream_t s) {
     hipLaunchKernelGGL(search_kernel_lds<MAXN>, dim3(grid), dim3(kWave), 0, s, P);
     return hipGetLastError();
 }
 
-hipError_t launch_class(int cls, const KParams& P, int grid, hipStream_t s) {
+hipError_t launch_class(int cls, const KParams* P, int grid, hipStream_t s) {
     switch (cls) {
         case 0: return launch_lds<64>(P, grid, s);
         case 1: return launch_lds<128>(P, grid, s);
@@ -115,15 +115,30 @@ uint64_t step_budget() {
     return 1ull << 40;
 }
 
-// Wall-clock limit of a launch in device ticks (GCL_TIME_LIMIT_MS; default
-// none): past it every worker stops with GCL_E_INTERNAL instead of running on.
-uint64_t time_limit_ticks() {
+// Wall-clock limits of a launch and of each search in it, in device ticks.
+// Past its limit a search stops with GCL_E_TIMEOUT, and so does any wait
+// inside it; past the launch's limit an idle worker leaves instead of waiting
+// for more work.  So no device wait is unbounded, whatever the protocol state.
+// The default allows a search 60 s + n^2 / 10^4 ms: a graph without
+// automorphisms and with an equitable initial partition (a random regular
+// graph) costs about n^2 splitter steps, a few seconds of the whole chip at
+// n = 10,000.
+TimeLimits time_limits(double sum_n2) {
+    const uint64_t tpm = device_wall_ticks_per_ms();
+    TimeLimits t;
     const char* e = getenv("GCL_TIME_LIMIT_MS");
-    if (!e || !*e) return ~0ull;
-    int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || rate_khz <= 0)
-        rate_khz = 100000;
-    return (uint64_t)strtoull(e, nullptr, 10) * (uint64_t)rate_khz;
+    if (e && *e) {
+        const uint64_t ms = strtoull(e, nullptr, 10);
+        if (ms == 0) return t;  // no limit
+        t.launch = t.base = ms * tpm;
+        t.per_n2 = 0;
+        return t;
+    }
+    t.base = 60000ull * tpm;
+    t.per_n2 = std::max<uint64_t>(1, tpm / 10000);
+    const double launch = 60000.0 * (double)tpm + sum_n2 * (double)t.per_n2;
+    t.launch = launch >= 1.8e19 ? ~0ull : (uint64_t)launch;
+    return t;
 }
 
 // Graphs with at least this many vertices publish their root children for
@@ -169,6 +184,7 @@ struct DeviceBatch {
     int64_t total_n = 0, total_nnz = 0;
     uint32_t max_aut = 1, ring = 1;
     DevBuf vtx, row, col, color, init, label, out, items, ctr, ctr_init;
+    DevBuf params;  // the classes' launch parameters (KParams), read by the kernels
     std::vector<Cls> classes;
     hipStream_t main = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, start = nullptr;
@@ -198,16 +214,24 @@ struct DeviceBatch {
     gcl_status create(const gcl_batch& b, uint32_t max_aut_, uint32_t ring_cap, const BatchTuning& t) {
         side = side_file_prefix();
         tune = t;
-        if (b.num_graphs >= 0) vtx_h.assign(b.vtx_off, b.vtx_off + b.num_graphs + 1);
         if (max_aut_ == 0) return fail(GCL_E_INVALID, "max_aut must be >= 1");
         G = b.num_graphs;
         if (G < 0) return fail(GCL_E_INVALID, "negative graph count");
+        // an empty batch may come with null arrays: nothing of it is read
+        if (G > 0) vtx_h.assign(b.vtx_off, b.vtx_off + G + 1);
+        else vtx_h.assign(1, 0);
         if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
         GCL_HIP(hipSetDevice(0));
         max_aut = max_aut_;
         ring = std::max(1u, std::min(max_aut, ring_cap));
         total_n = G > 0 ? b.vtx_off[G] : 0;
         total_nnz = G > 0 ? b.row_ptr[total_n] : 0;
+        double sum_n2 = 0.0;
+        for (int32_t g = 0; g < G; ++g) {
+            const double n = (double)(b.vtx_off[g + 1] - b.vtx_off[g]);
+            sum_n2 += n * n;
+        }
+        const TimeLimits lim = time_limits(sum_n2);
 
         // size classes, larger graphs first within a class
         std::vector<std::vector<int32_t>> items_h(kClasses + 1);
@@ -259,8 +283,11 @@ struct DeviceBatch {
         GCL_HIP(hipMalloc(&items.p, sizeof(int32_t) * (size_t)std::max(G, 1)));
         GCL_HIP(hipMalloc(&ctr.p, sizeof(unsigned int) * 64));
         GCL_HIP(hipMalloc(&ctr_init.p, sizeof(unsigned int) * 64));
-        GCL_HIP(hipMemcpy(vtx.p, b.vtx_off, sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice));
-        GCL_HIP(hipMemcpy(row.p, b.row_ptr, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice));
+        GCL_HIP(hipMemcpy(vtx.p, vtx_h.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice));
+        if (G > 0)
+            GCL_HIP(hipMemcpy(row.p, b.row_ptr, sizeof(int64_t) * (size_t)(total_n + 1), hipMemcpyHostToDevice));
+        else
+            GCL_HIP(hipMemset(row.p, 0, sizeof(int64_t)));
         if (total_nnz > 0)
             GCL_HIP(hipMemcpy(col.p, b.col, sizeof(int32_t) * (size_t)total_nnz, hipMemcpyHostToDevice));
         if (any_color) {
@@ -322,11 +349,15 @@ struct DeviceBatch {
                 // holds every root child unless that exceeds the memory budget
                 spec_records = (uint32_t)std::min<uint64_t>({(uint64_t)cls_spec_big[cls], (uint64_t)C.grid,
                                                              (uint64_t)kMaxRecords});
-                const uint64_t per_slot = make_spec_layout(maxn, 2).total - make_spec_layout(maxn, 1).total + 64;
+                // item metadata for every root child of a class-sized graph;
+                // the partition payload as the budget allows (a record holds
+                // items in proportion to 1/n of the graph that publishes)
                 const uint64_t budget = 8ull << 30;
-                spec_slots = (uint32_t)std::min<uint64_t>(
-                    maxn, std::max<uint64_t>(64, budget / std::max<uint32_t>(spec_records, 1) / per_slot));
-                spec_stride = make_spec_layout(maxn, spec_slots).total;
+                spec_slots = maxn;
+                spec_stride = std::min<uint64_t>(spec_record_bytes(maxn, spec_slots, spec_slots),
+                                                 std::max<uint64_t>(spec_record_bytes(maxn, spec_slots, 64),
+                                                                    budget / std::max<uint32_t>(spec_records, 1)));
+                spec_stride = align256(spec_stride);
                 GCL_HIP(hipMalloc(&C.spec.p, (size_t)(spec_stride * spec_records)));
                 GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(spec_stride * spec_records)));
                 const uint32_t nwr = (spec_records + 63) / 64;
@@ -372,7 +403,9 @@ struct DeviceBatch {
             P.spec_stride = spec_stride;
             P.spec_slots = spec_slots;
             P.spec_min_n = spec ? spec_min : 0u;
-            P.time_limit = time_limit_ticks();
+            P.time_limit = lim.launch;
+            P.graph_ticks_base = lim.base;
+            P.graph_ticks_per_n2 = lim.per_n2;
             P.spec_records = spec_records;
             P.slot_free = spec ? (unsigned long long*)C.board.p : nullptr;
             P.board_mask = spec ? (unsigned long long*)C.board.p + (spec_records + 63) / 64 : nullptr;
@@ -392,6 +425,12 @@ struct DeviceBatch {
         for (auto& c : classes)
             if (c.P.spec_min_n) init_ctr[32 + c.cls] = cls_spec_big[c.cls];
         GCL_HIP(hipMemcpy(ctr_init.p, init_ctr.data(), sizeof(unsigned int) * 64, hipMemcpyHostToDevice));
+        if (!classes.empty()) {
+            std::vector<KParams> ph;
+            for (auto& c : classes) ph.push_back(c.P);
+            GCL_HIP(hipMalloc(&params.p, sizeof(KParams) * ph.size()));
+            GCL_HIP(hipMemcpy(params.p, ph.data(), sizeof(KParams) * ph.size(), hipMemcpyHostToDevice));
+        }
         return GCL_OK;
     }
 
@@ -410,9 +449,10 @@ struct DeviceBatch {
         // same queue would hold that launch back
         for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
             Cls& c = *it;
+            const KParams* dP = (const KParams*)params.p + (it.base() - 1 - classes.begin());
             if ((e = hipStreamWaitEvent(c.stream, start, 0)) != hipSuccess) return e;
             if (timed && (e = hipEventRecord(c.t0, c.stream)) != hipSuccess) return e;
-            if ((e = launch_class(c.cls, c.P, (int)c.grid, c.stream)) != hipSuccess) return e;
+            if ((e = launch_class(c.cls, dP, (int)c.grid, c.stream)) != hipSuccess) return e;
             if (timed && (e = hipEventRecord(c.t1, c.stream)) != hipSuccess) return e;
             if ((e = hipEventRecord(c.done, c.stream)) != hipSuccess) return e;
         }
@@ -516,6 +556,7 @@ struct DeviceBatch {
                 o.nodes += res[(size_t)g].nodes;
                 o.splitter_steps += res[(size_t)g].splitter_steps;
                 o.csr_bytes += res[(size_t)g].csr_bytes;
+                o.alg_bytes += res[(size_t)g].csr_bytes + res[(size_t)g].part_bytes;
             }
         }
         return GCL_OK;
@@ -557,13 +598,14 @@ struct DeviceBatch {
         GCL_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
         const double ns_per_tick = rate_khz > 0 ? 1e6 / (double)rate_khz : 10.0;
         for (auto& r : res) r.device_ns = (uint64_t)((double)r.device_ns * ns_per_tick);
-        uint64_t nodes = 0, auts = 0, steps = 0, bytes = 0;
+        uint64_t nodes = 0, auts = 0, steps = 0, bytes = 0, alg = 0;
         gcl_status worst = GCL_OK;
         for (int32_t g = 0; g < G; ++g) {
             nodes += res[(size_t)g].nodes;
             auts += res[(size_t)g].automorphisms;
             steps += res[(size_t)g].splitter_steps;
             bytes += res[(size_t)g].csr_bytes;
+            alg += res[(size_t)g].csr_bytes + res[(size_t)g].part_bytes;
             if (res[(size_t)g].status != 0 && worst == GCL_OK) {
                 worst = (gcl_status)res[(size_t)g].status;
                 set_error("graph %d: device search failed (%s)", g, gcl_status_string(worst));
@@ -575,6 +617,7 @@ struct DeviceBatch {
             st->automorphisms = auts;
             st->splitter_steps = steps;
             st->csr_bytes = bytes;
+            st->alg_bytes = alg;
             st->workers = workers;
         }
         return worst;
@@ -593,12 +636,12 @@ struct DeviceBatch {
 // kernels with system-scope loads; results written back the same way), and
 // the device arrays the claiming worker copies a request into.
 template <int MAXN>
-hipError_t launch_serve_lds(const KParams& P, int grid, hipStream_t s) {
+hipError_t launch_serve_lds(const KParams* P, int grid, hipStream_t s) {
     hipLaunchKernelGGL(serve_kernel_lds<MAXN>, dim3(grid), dim3(kWave), 0, s, P);
     return hipGetLastError();
 }
 
-hipError_t launch_serve(int cls, const KParams& P, int grid, hipStream_t s) {
+hipError_t launch_serve(int cls, const KParams* P, int grid, hipStream_t s) {
     switch (cls) {
         case 0: return launch_serve_lds<64>(P, grid, s);
         case 1: return launch_serve_lds<128>(P, grid, s);
@@ -652,6 +695,7 @@ struct ServeEngine {
         uint32_t cq_head = 0;  // completions consumed
     };
     std::vector<Cls> classes;
+    DevBuf params;  // the serving kernels' launch parameters (KParams)
     bool running = false;
 
     ~ServeEngine() {
@@ -686,7 +730,8 @@ struct ServeEngine {
         };
         std::vector<int> ks = {class_for(std::min<uint32_t>((uint32_t)cfg.max_n, 1024u))};
         if (cfg.max_n > 1024) ks.push_back(class_for((uint32_t)cfg.max_n));
-        const double share[2] = {ks.size() == 1 ? 0.9 : 0.7, 0.2};
+        const double big = std::min(0.8, std::max(0.05, cfg.large_share));
+        const double share[2] = {ks.size() == 1 ? 0.9 : 0.9 - big, big};
         const uint32_t smin = spec_min_n(cfg.spec_min_n);
         std::vector<KParams> params;
         for (size_t ki = 0; ki < ks.size(); ++ki) {
@@ -705,10 +750,15 @@ struct ServeEngine {
             GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel(k), kWave, 0));
             if (per_cu <= 0) per_cu = 1;
             int64_t grid = (int64_t)((double)per_cu * prop.multiProcessorCount * share[ki]);
-            grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)(cfg.scratch_bytes_per_class / L.total)));
+            const uint64_t scratch_budget = ki == 0 ? cfg.scratch_bytes_per_class : cfg.scratch_bytes_large;
+            grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)(scratch_budget / L.total)));
             C.grid = grid;
+            // request slots: twice the workers in the first class; a large
+            // graph keeps many workers busy (its root children), so the large
+            // class holds a few dozen requests at a time
             C.slots = cfg.slots > 0 ? (uint32_t)cfg.slots
-                                    : (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(64, 2 * grid));
+                      : ki == 0     ? (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(64, 2 * grid))
+                                    : (uint32_t)std::min<int64_t>(64, std::max<int64_t>(8, grid));
             C.ring = 1;
             while (C.ring < C.slots) C.ring <<= 1;
             const size_t S = C.slots, n1 = (size_t)C.maxn + 1;
@@ -760,13 +810,23 @@ struct ServeEngine {
             P.arena_cap = arena;
             P.levels = levels;
             P.step_budget = step_budget();
+            // a serving worker runs until the host stops the queue (or its
+            // lifetime passes); each request's search has its own limit
+            const TimeLimits lim = time_limits(0.0);
             P.time_limit = cfg.lifetime_ticks;
+            P.graph_ticks_base = lim.base;
+            P.graph_ticks_per_n2 = lim.per_n2;
             if (smin > 0 && C.maxn >= smin) {
-                const uint32_t records = (uint32_t)std::min<int64_t>(grid, (int64_t)kMaxRecords);
-                const uint64_t per_item = make_spec_layout(C.maxn, 2).total - make_spec_layout(C.maxn, 1).total + 64;
-                const uint32_t spec_slots = (uint32_t)std::min<uint64_t>(
-                    C.maxn, std::max<uint64_t>(64, cfg.spec_bytes_per_class / records / per_item));
-                const uint64_t stride = make_spec_layout(C.maxn, spec_slots).total;
+                // one record per worker in the first class; the large class
+                // keeps fewer, larger ones, so that each holds every root
+                // child of the graphs that publish
+                const uint32_t records =
+                    (uint32_t)std::min<int64_t>(grid, ki == 0 ? (int64_t)kMaxRecords : (int64_t)C.slots);
+                const uint64_t budget = ki == 0 ? cfg.spec_bytes_per_class : cfg.spec_bytes_large;
+                const uint32_t spec_slots = C.maxn;
+                const uint64_t stride = align256(std::min<uint64_t>(
+                    spec_record_bytes(C.maxn, spec_slots, spec_slots),
+                    std::max<uint64_t>(spec_record_bytes(C.maxn, spec_slots, 64), budget / records)));
                 GCL_HIP(hipMalloc(&C.spec.p, (size_t)(stride * records)));
                 GCL_HIP(hipMemset(C.spec.p, 0, (size_t)(stride * records)));
                 const uint32_t nwr = (records + 63) / 64;
@@ -817,10 +877,13 @@ struct ServeEngine {
             params.push_back(P);
             classes.push_back(std::move(C));
         }
+        GCL_HIP(hipMalloc(&this->params.p, sizeof(KParams) * params.size()));
+        GCL_HIP(hipMemcpy(this->params.p, params.data(), sizeof(KParams) * params.size(), hipMemcpyHostToDevice));
         // everything is in place before the first persistent launch
         GCL_HIP(hipDeviceSynchronize());
         for (size_t ki = 0; ki < classes.size(); ++ki) {
-            hipError_t e = launch_serve(classes[ki].cls, params[ki], (int)classes[ki].grid, classes[ki].stream);
+            const KParams* dP = (const KParams*)this->params.p + ki;
+            hipError_t e = launch_serve(classes[ki].cls, dP, (int)classes[ki].grid, classes[ki].stream);
             if (e != hipSuccess) return fail(GCL_E_DEVICE, "serving kernel launch failed: %s", hipGetErrorString(e));
             running = true;
         }
@@ -841,11 +904,15 @@ struct ServeEngine {
         return r;
     }
 
+    // GCL_OK while every serving kernel runs; GCL_E_TIMEOUT once one has
+    // returned (its lifetime passed: it takes no more requests), GCL_E_DEVICE
+    // when one failed.  Only meaningful before stop().
     gcl_status check() {
         for (auto& c : classes) {
             const hipError_t e = hipStreamQuery(c.stream);
-            if (e != hipSuccess && e != hipErrorNotReady)
-                return fail(GCL_E_DEVICE, "serving kernel failed: %s", hipGetErrorString(e));
+            if (e == hipErrorNotReady) continue;
+            if (e == hipSuccess) return fail(GCL_E_TIMEOUT, "serving kernel stopped: the server's lifetime passed");
+            return fail(GCL_E_DEVICE, "serving kernel failed: %s", hipGetErrorString(e));
         }
         return GCL_OK;
     }

@@ -205,6 +205,7 @@ const char* gcl_status_string(gcl_status s) {
         case GCL_E_CAPACITY: return "capacity exceeded";
         case GCL_E_INTERNAL: return "internal error";
         case GCL_E_BUSY: return "busy";
+        case GCL_E_TIMEOUT: return "time limit exceeded";
     }
     return "unknown status";
 }
@@ -278,7 +279,10 @@ gcl_status gcl_read_dimacs(const char* path, gcl_graph** out) {
         } else if (type == 'n') {
             long long vid = 0, col = 0;
             iss >> vid >> col;
-            if (vid == 0) continue;
+            // the reference reads the id as unsigned (CSRGraph.cpp:62-68): 0
+            // is skipped and a negative id wraps past n, so its colour is
+            // ignored by the range check below (CSRGraph.cpp:86-90)
+            if (vid < 1) continue;
             colors.push_back({vid - 1, (int32_t)col});
         } else if (type == 'e') {
             long long u = 0, v = 0;
@@ -306,7 +310,7 @@ gcl_status gcl_read_dimacs(const char* path, gcl_graph** out) {
     if (!colors.empty()) {
         g.color.assign((size_t)n, 0);
         for (auto& c : colors)
-            if (c.first < n) g.color[(size_t)c.first] = c.second;
+            if (c.first >= 0 && c.first < n) g.color[(size_t)c.first] = c.second;
     }
     return graph_to_owned(std::move(g), out);
 }

@@ -38,6 +38,24 @@ uint64_t graph_footprint(int64_t n, int64_t nnz) {
     return 8ull * (uint64_t)(n + 1) + 4ull * (uint64_t)nnz + 4ull * (uint64_t)n + sizeof(DeviceGraphResult);
 }
 
+// Estimated search cost in splitter steps.  When every vertex has the same
+// degree and the same colour, the initial partition is already equitable:
+// refinement at the root splits nothing, and a graph without automorphisms
+// (a random regular graph) has n root children that each refine to about n
+// splitter steps, n^2 in all.  Otherwise degree refinement splits the root
+// and the search is short; it is costed as n.
+uint64_t search_cost_estimate(const gcl_csr& g) {
+    const uint64_t n = (uint64_t)g.n;
+    if (g.n <= 1) return n;
+    const int32_t d0 = g.row_ptr[1] - g.row_ptr[0];
+    for (int32_t v = 1; v < g.n; ++v)
+        if (g.row_ptr[v + 1] - g.row_ptr[v] != d0) return n;
+    if (g.color)
+        for (int32_t v = 1; v < g.n; ++v)
+            if (g.color[v] != g.color[0]) return n;
+    return n * n;
+}
+
 Graph copy_graph(const gcl_csr& c) {
     Graph g;
     g.n = c.n;
@@ -100,9 +118,11 @@ struct Server {
     gcl_status create(const gcl_server_config& c) {
         cfg = c;
         if (cfg.max_n < 1 || cfg.max_n > kMaxVertices || cfg.max_graph_nnz < 0 || cfg.max_aut == 0 ||
-            cfg.queue_bytes == 0 || cfg.slots < 0 || cfg.lifetime_ms < 0)
+            cfg.queue_bytes == 0 || cfg.slots < 0 || cfg.lifetime_ms < 0 || cfg.large_share_pct < 0 ||
+            cfg.large_share_pct > 80 || (cfg.large_share_pct > 0 && cfg.large_share_pct < 5))
             return fail(GCL_E_INVALID, "invalid server configuration");
         ServeConfig sc;
+        sc.large_share = (cfg.large_share_pct > 0 ? cfg.large_share_pct : 40) / 100.0;
         sc.max_n = cfg.max_n;
         sc.max_graph_nnz = cfg.max_graph_nnz;
         sc.max_aut = cfg.max_aut;
@@ -130,10 +150,25 @@ struct Server {
         const gcl_csr* gs[2] = {a, b};
         const int ng = b ? 2 : 1;
         uint64_t fp = 0;
+        auto refuse = [&](const char* why, int i, unsigned long long x, unsigned long long cap) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                ++st.refused_capacity;
+            }
+            return fail(GCL_E_CAPACITY, "graph (n=%d, nnz=%lld): %s %llu > %llu", gs[i]->n, (long long)gs[i]->nnz,
+                        why, x, cap);
+        };
         for (int i = 0; i < ng; ++i) {
+            if (gs[i]->n > cfg.max_n)
+                return refuse("n above the server's max_n", i, (unsigned long long)gs[i]->n,
+                              (unsigned long long)cfg.max_n);
             if (serve_class_of(eng, gs[i]->n, gs[i]->nnz) < 0)
-                return fail(GCL_E_CAPACITY, "graph (n=%d, nnz=%lld) exceeds the server's max_n=%d / max_graph_nnz=%lld",
-                            gs[i]->n, (long long)gs[i]->nnz, cfg.max_n, (long long)cfg.max_graph_nnz);
+                return refuse("edges above the largest class's", i, (unsigned long long)gs[i]->nnz,
+                              (unsigned long long)cfg.max_graph_nnz);
+            const uint64_t cost = search_cost_estimate(*gs[i]);
+            if (cfg.max_search_cost > 0 && cost > cfg.max_search_cost)
+                return refuse("estimated search cost (splitter steps) above max_search_cost", i,
+                              (unsigned long long)cost, (unsigned long long)cfg.max_search_cost);
             fp += graph_footprint(gs[i]->n, gs[i]->nnz);
         }
         if (fp > cfg.queue_bytes) return fail(GCL_E_CAPACITY, "request footprint exceeds queue_bytes");
@@ -141,7 +176,7 @@ struct Server {
             // the footprint is reserved with the check, so concurrent
             // submitters cannot together pass queue_bytes
             std::lock_guard<std::mutex> lk(mu);
-            if (fatal != GCL_OK) return fail(fatal, "server stopped after a device error");
+            if (fatal != GCL_OK) return fail(fatal, "server stopped: %s", gcl_status_string(fatal));
             if (queued_bytes + fp > cfg.queue_bytes) {
                 ++st.refused_busy;
                 return fail(GCL_E_BUSY, "server queue full");
@@ -340,10 +375,19 @@ struct Server {
             publish_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(c2 - c1).count();
             bool inflight = false;
             for (auto& C : cls) inflight = inflight || C.busy > 0;
-            if (inflight && Clock::now() - last_check > std::chrono::milliseconds(20)) {
+            bool waiting;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                waiting = !pending.empty();
+            }
+            if ((inflight || waiting) && Clock::now() - last_check > std::chrono::milliseconds(20)) {
                 last_check = Clock::now();
-                if (serve_check(eng) != GCL_OK) {
-                    fail_all(GCL_E_DEVICE);
+                // a serving kernel that failed, or that returned because its
+                // lifetime passed, completes nothing more: the requests in
+                // flight and queued fail, and later submissions are refused
+                const gcl_status cs = serve_check(eng);
+                if (cs != GCL_OK) {
+                    fail_all(cs);
                     return;
                 }
             }
@@ -436,7 +480,7 @@ struct Server {
             {
                 std::lock_guard<std::mutex> lk(mu);
                 if (fatal != GCL_OK && st.completed == st.submitted && done.empty() && got < count)
-                    return fail(fatal, "server stopped after a device error");
+                    return fail(fatal, "server stopped: %s", gcl_status_string(fatal));
             }
             if (k == 0 && next < count && !blocked) {
                 // sleep until the next arrival (busy-wait the last 200 us)
@@ -477,6 +521,8 @@ void gcl_server_config_default(gcl_server_config* c) {
     c->slots = 0;
     c->queue_bytes = 1ull << 30;
     c->lifetime_ms = 0;
+    c->large_share_pct = 40;
+    c->max_search_cost = 150000000ull;
 }
 
 gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out) {

@@ -38,6 +38,9 @@ constexpr int kWave = 64;
 constexpr uint32_t kMaxRecords = 4096;  // speculation records per launch (64 mask words)
 constexpr uint32_t kNoKey = 0u;
 constexpr uint32_t kFailedCode = 0xFFFFFFFFu;  // speculation item whose refinement failed
+// Set in an item's code word when the helper left its partition out: the
+// code alone rejects the child (see help_item).
+constexpr uint32_t kCodeOnly = 0x40000000u;
 
 struct LevelRec {
     uint32_t tc_first, tc_len, cursor, arena_off;
@@ -109,11 +112,19 @@ struct SpecHeader {
     uint32_t next;      // helper claim counter
     uint32_t tc_first;  // first position of the root target cell
     uint32_t n;
-    uint32_t pad[2];
+    int32_t first1;     // the first path's code at level 2
+    int32_t canon1;     // the best path's code at level 2 (only grows; the owner refreshes it)
 };
 
+// A record has a fixed part, sized by the launch's largest n and its item
+// capacity `slots` (header, root partition, per-item vertex, state, code and
+// counters), and a payload of refined partitions, laid out by the n of the
+// graph that publishes (its element vector, then its cell-boundary bitmap, per
+// item).  A record of a given size thus holds the root children of a graph in
+// proportion to 1/n: every root child of a 10,000-vertex graph fits in the
+// records of a class whose largest graph has 50,000 vertices.
 struct SpecLayout {
-    uint64_t header, root_elem, root_lasts, vert, state, code, ncells, steps, elem, lasts, total;
+    uint64_t header, root_elem, root_lasts, vert, state, code, ncells, steps, payload;
     uint32_t slots;
 };
 
@@ -130,11 +141,27 @@ __host__ __device__ inline SpecLayout make_spec_layout(uint32_t maxn, uint32_t s
     L.state = o; o = align256(o + 4 * (uint64_t)slots);
     L.code = o; o = align256(o + 4 * (uint64_t)slots);
     L.ncells = o; o = align256(o + 4 * (uint64_t)slots);
-    L.steps = o; o = align256(o + 16 * (uint64_t)slots);  // (splitter steps, CSR bytes) per item
-    L.elem = o; o = align256(o + 2 * n1 * slots);
-    L.lasts = o; o = align256(o + 8 * nw1 * slots);
-    L.total = o;
+    // per item: splitter steps, CSR bytes, partition bytes (and one spare word)
+    L.steps = o; o = align256(o + 32 * (uint64_t)slots);
+    L.payload = o;
     return L;
+}
+__host__ __device__ inline uint64_t align128(uint64_t x) { return (x + 127) & ~uint64_t(127); }
+// payload bytes of one item of a graph with n vertices, and the offset of its
+// bitmap inside the item
+__host__ __device__ inline uint64_t spec_item_lasts(uint32_t n) { return align128(2 * ((uint64_t)n + 2)); }
+__host__ __device__ inline uint64_t spec_item_bytes(uint32_t n) {
+    return spec_item_lasts(n) + align128(8 * ((uint64_t)n / 64 + 2));
+}
+// items of a graph with n vertices that a record of `stride` bytes holds
+__host__ __device__ inline uint32_t spec_item_cap(const SpecLayout& L, uint64_t stride, uint32_t n) {
+    const uint64_t room = stride > L.payload ? (stride - L.payload) / spec_item_bytes(n) : 0;
+    return (uint32_t)(room < L.slots ? room : L.slots);
+}
+// the record size with metadata for `slots` items that holds the partitions
+// of `items` of them for a graph with maxn vertices
+__host__ __device__ inline uint64_t spec_record_bytes(uint32_t maxn, uint32_t slots, uint32_t items) {
+    return align256(make_spec_layout(maxn, slots).payload + spec_item_bytes(maxn) * (uint64_t)items);
 }
 
 // Serving mode: requests published by the host while the kernel runs.  The
@@ -197,7 +224,11 @@ struct KParams {
     unsigned long long* slot_free;   // bit r set while record r is free
     unsigned long long* board_mask;  // bit r set while record r is published
     unsigned int* pending;      // running graphs that may still publish
-    uint64_t time_limit;        // wall-clock ticks after which a worker gives up (waits, search)
+    uint64_t time_limit;        // wall-clock ticks a worker runs at most (its loop ends)
+    // a search (and every wait inside it) gives up with GCL_E_TIMEOUT after
+    // graph_ticks_base + graph_ticks_per_n2 * n^2 ticks, or at the worker's
+    // end, whichever comes first
+    uint64_t graph_ticks_base, graph_ticks_per_n2;
     DiagRecord* diag;           // per-worker records (grid entries), or NULL
     uint8_t* aut_pool;          // automorphism rings, one per item (aut_stride bytes), or NULL
     uint64_t aut_stride;
@@ -444,6 +475,14 @@ struct HotG {
 struct Ctx {
     uint32_t n, nw, ncells, nactive;
     uint64_t steps, csr_bytes;
+    // algorithmic bytes beyond the CSR reads: 8 per non-singleton cell hashed
+    // under a non-singleton splitter, 8 per vertex of a cell that splits, 8n
+    // per refinement (the element vector loaded and stored once)
+    uint64_t part_bytes;
+    // per lane: sizes of the split cells this lane regrouped or owned since
+    // the last flush into part_bytes (one vector add per split instead of a
+    // lane read; at most 2^14 steps between flushes keeps it below 2^32)
+    uint32_t lane_split;
     const int32_t* col;
     const int64_t* rp;  // global row offsets and rp[0] (rows outside LDS)
     int64_t rb;
@@ -469,13 +508,15 @@ struct Worker {
     uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
     uint64_t arena_cap;
     uint32_t ring, levels;
-    uint64_t step_budget, deadline;
+    uint64_t step_budget;
+    uint64_t life;      // wall-clock tick at which this worker's loop ends
+    uint64_t deadline;  // tick at which the current search gives up (<= life)
     // search state (reference src/sequential.h:24-58)
     uint32_t level, seq_size, tc_level_reached, eqlevfirst, first_seq_size, best_seq_size;
     int eqlevcanon, comp_canon, canon_level, code;
     bool first_discovered;
     int64_t num_aut;
-    uint64_t nodes, steps, csr_bytes, budget;
+    uint64_t nodes, steps, csr_bytes, part_bytes, budget;
     int32_t status;
     // speculation (owner side)
     bool root_changed;       // the partition held at level 1 is not the root's (own helping)
@@ -748,6 +789,7 @@ struct Worker {
         // boundary hash chain in position order (399-409): x = f+k-1 grows
         // with f; the hint is the candidate of the last split that has one
         const uint32_t x = f + k - 1;
+        if (own) X.lane_split += L;
         if (nsplit == 1) {
             const int j = __builtin_ctzll(om);
             longcode = GCL_MASH(longcode, readlane(x, j));
@@ -1138,11 +1180,14 @@ struct Worker {
                 big = !small;
             }
             // the 4-wide network when no small cell here has more than 4 members
+            uint32_t nf = 1;
             if (ballot(small && c - f + 1 > 4)) {
-                if (small) nfrag_extra += regroup_small<8>(f, c, s) - 1;
+                if (small) nf = regroup_small<8>(f, c, s);
             } else {
-                if (small) nfrag_extra += regroup_small<4>(f, c, s) - 1;
+                if (small) nf = regroup_small<4>(f, c, s);
             }
+            nfrag_extra += nf - 1;
+            if (nf > 1) X.lane_split += c - f + 1;
             const uint64_t bm = ballot(big);
             if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
             nbig += (uint32_t)__popcll(bm);
@@ -1155,7 +1200,10 @@ struct Worker {
             const uint32_t c = uni((uint32_t)H.tcl(j));
             const uint32_t f = uni((uint32_t)H.cfirst(c));
             const uint32_t L = c - f + 1;
-            if (s > 1 && sparse_split(X, f, c, L, ntv)) continue;
+            if (s > 1 && sparse_split(X, f, c, L, ntv)) {
+                X.part_bytes += 8ull * L;
+                continue;
+            }
             uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
@@ -1169,6 +1217,7 @@ struct Worker {
             kmin = uni(wave_min(kmin));
             kmax = uni(wave_max(kmax));
             if (kmin == kmax) continue;  // uniform: nothing moves
+            X.part_bytes += 8ull * L;
             const uint32_t R = kmax - kmin + 1;
             for (uint32_t base = 0; base < L; base += kWave) {
                 const uint32_t i = base + (uint32_t)l;
@@ -1245,6 +1294,7 @@ struct Worker {
         H.sync();
         // hash chain / active rules in position order (reference 253-468)
         if (s > 1) {
+            X.part_bytes += 8ull * nns;  // the non-singleton cell headers hashed
             hash_pass_parallel(X, nns, longcode, hint);
         } else {
             // single-vertex splitter: cells that did not split add nothing to
@@ -1312,6 +1362,15 @@ struct Worker {
         H.sync();
     }
 
+    // 8 bytes per vertex of the split cells counted in the lanes (every lane
+    // active; each lane's count is below 2^32, and the halves' sums fit too)
+    __device__ __forceinline__ static void flush_split_bytes(Ctx& X) {
+        const uint32_t lo = uni(wave_sum(X.lane_split & 0xFFFFu));
+        const uint32_t hi = uni(wave_sum(X.lane_split >> 16));
+        X.part_bytes += 8ull * ((uint64_t)lo + ((uint64_t)hi << 16));
+        X.lane_split = 0;
+    }
+
     // Partition_s_refinement (reference src/partition_refinement.cpp:150-485)
     __device__ int refine(int32_t init_active) {
         const int l = lane_id();
@@ -1322,6 +1381,8 @@ struct Worker {
         X.nactive = uni(nactive);
         X.steps = uni(steps);
         X.csr_bytes = uni(csr_bytes);
+        X.part_bytes = uni(part_bytes) + 8ull * X.n;
+        X.lane_split = 0;
         X.col = uni(col);
         X.rp = uni(H.rows());
         X.rb = Hot::kRowLds ? 0 : uni(gload(X.rp));
@@ -1386,7 +1447,9 @@ struct Worker {
             H.sync();
             if (e == p && deg <= (uint32_t)kWave) singleton_step(X, u, deg, longcode, hint);
             else general_step(X, p, e, longcode, hint);
+            if ((X.steps & 0x3FFFu) == 0) flush_split_bytes(X);
         }
+        flush_split_bytes(X);
         budget = bud;
         if (X.nactive) {
             for (uint32_t w = (uint32_t)l; w < X.nw; w += kWave) H.active()[w] = 0;
@@ -1397,6 +1460,7 @@ struct Worker {
         nactive = 0;
         steps = X.steps;
         csr_bytes = X.csr_bytes;
+        part_bytes = X.part_bytes;
         longcode = GCL_MASH(longcode, X.ncells);
         return GCL_CLEANUP(longcode);
     }
@@ -1954,7 +2018,7 @@ struct Worker {
         SpecHeader* hd = spec_header();
         const uint32_t epoch = hd->epoch + 1;
         spec_epoch = epoch;
-        const uint32_t count = min(cnt, SL.slots);
+        const uint32_t count = min(cnt, spec_item_cap(SL, P.spec_stride, n));
         spec_count = count;
         spec_cursor = 0;
         // root partition and item list
@@ -1978,6 +2042,8 @@ struct Worker {
             hd->count = count;
             hd->tc_first = r.tc_first;
             hd->n = n;
+            hd->first1 = firstcode[1];
+            __hip_atomic_store(&hd->canon1, canon_code[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&hd->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         // data and header before the item states that make items claimable
@@ -2016,6 +2082,10 @@ struct Worker {
         }
         if (spec_cursor >= spec_count || (uint32_t)vert[spec_cursor] != v) return kNotTaken;
         const uint32_t j = spec_cursor++;
+        // the best path's code at level 2 as it is now: helpers that finish
+        // later leave out the partitions of children it rejects
+        if (lane_id() == 0)
+            __hip_atomic_store(&spec_header()->canon1, canon_code[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t won = 0;
         if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 3u) ? 1u : 0u;
         won = __shfl(won, 0, kWave);
@@ -2024,44 +2094,53 @@ struct Worker {
         const uint64_t tw = wall_clock64();
         while (agent_peek(&state[j]) != (E | 2u)) {
             if (wall_clock64() > deadline) {
-                status = GCL_E_INTERNAL;  // a helper did not finish in time
+                status = GCL_E_TIMEOUT;  // a helper did not finish in time
                 return kNotTaken;
             }
             if (!spec_help_own(P, E)) __builtin_amdgcn_s_sleep(2);
         }
         dg_wait += wall_clock64() - tw;
         agent_acquire();
-        const uint32_t hc = uni(((const uint32_t*)(spec_rec + SL.code))[j]);
+        const uint32_t hw = uni(((const uint32_t*)(spec_rec + SL.code))[j]);
+        if (hw == kFailedCode) {
+            status = GCL_E_INTERNAL;  // the helper's refinement failed
+            return kNotTaken;
+        }
+        const uint32_t hc = hw & ~kCodeOnly;
+        const uint64_t* cnt_rec = (const uint64_t*)(spec_rec + SL.steps) + 4ull * j;
         // A child whose code is below the best path's at this level and
         // differs from the first path's is classified kBad before its
         // partition is looked at (classify_node, reference
         // src/sequential.cpp:396-624), and the backtrack that follows puts
-        // back the root partition this worker still holds.  In the root
-        // state (level 1, eqlevcanon 1, comp_canon 0: every root child after
-        // the first path returns to it) such a child costs its counters only,
-        // unless this worker refined a child of its own record meanwhile (the
-        // partition it holds is then that child's).
-        if (hc != kFailedCode && !root_changed && level == 1 && seq_size == 0 && eqlevfirst <= 1 &&
-            eqlevcanon == 1 && comp_canon == 0) {
+        // back the root partition.  In the root state (level 1, eqlevcanon
+        // 1, comp_canon 0: every root child after the first path returns to
+        // it) such a child costs its counters only.  When this worker refined
+        // a child of its own record meanwhile, the partition it holds is that
+        // child's, and the level-1 snapshot puts the root's back.
+        if (level == 1 && seq_size == 0 && eqlevfirst <= 1 && eqlevcanon == 1 && comp_canon == 0) {
             const bool eq_first = eqlevfirst == 1 && (int)hc == uni(firstcode[1]);
             if (!eq_first && (int)hc < uni(canon_code[1])) {
-                steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
-                csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
+                steps += cnt_rec[0];
+                csr_bytes += cnt_rec[1];
+                part_bytes += cnt_rec[2];
+                if (root_changed) restore_level(1);
                 tc_level_reached = 1;  // what backtrack(1) leaves (the rest is unchanged)
                 return kRejected;
             }
         }
-        if (hc == kFailedCode) {
-            status = GCL_E_INTERNAL;  // the helper's refinement failed
-            return kNotTaken;
-        }
-        const ix_t* ie = (const ix_t*)(spec_rec + SL.elem) + (uint64_t)j * (P_n1());
-        const uint64_t* il = (const uint64_t*)(spec_rec + SL.lasts) + (uint64_t)j * (P_nw1());
+        // a helper leaves the partition out only for a child that the code
+        // rejects; should this worker not be in the state that rejects it,
+        // the child is refined here
+        if (hw & kCodeOnly) return kNotTaken;
+        const uint8_t* item = spec_rec + SL.payload + (uint64_t)j * spec_item_bytes(n);
+        const ix_t* ie = (const ix_t*)item;
+        const uint64_t* il = (const uint64_t*)(item + spec_item_lasts(n));
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) H.lasts()[w] = il[w];
         code = (int)hc;
-        steps += ((const uint64_t*)(spec_rec + SL.steps))[2 * j];
-        csr_bytes += ((const uint64_t*)(spec_rec + SL.steps))[2 * j + 1];
+        steps += cnt_rec[0];
+        csr_bytes += cnt_rec[1];
+        part_bytes += cnt_rec[2];
         H.sync();
         positions_from_elem();
         rebuild_from_lasts();
@@ -2082,19 +2161,17 @@ struct Worker {
         }
         idx = uni(readlane(idx, 0));
         if (idx == 0xFFFFFFFFu) return false;
-        const uint64_t st0 = steps, by0 = csr_bytes, bud0 = budget;
+        const uint64_t st0 = steps, by0 = csr_bytes, pb0 = part_bytes, bud0 = budget;
         const int32_t s0 = status;
         root_changed = true;  // help_item works in this worker's partition
         help_item(P, spec_rec, idx, E);
         steps = st0;
         csr_bytes = by0;
+        part_bytes = pb0;
         budget = bud0;
         status = s0;
         return true;
     }
-    __device__ uint64_t P_n1() const { return (uint64_t)spec_maxn + 2; }
-    __device__ uint64_t P_nw1() const { return (uint64_t)spec_maxn / 64 + 2; }
-    uint32_t spec_maxn;
 
     // Ends the publication: withdraw unclaimed items, wait for running ones.
     __device__ void spec_finish(const KParams& P) {
@@ -2105,13 +2182,22 @@ struct Worker {
         if (lane_id() == 0)
             atomicAnd(&P.board_mask[spec_slot >> 6], ~(unsigned long long)(1ull << (spec_slot & 63)));
         group_sync();
+        bool held = false;
         for (uint32_t j = (uint32_t)lane_id(); j < spec_count; j += kWave) {
             agent_cas(&state[j], E | 0u, E | 3u);
-            while (agent_peek(&state[j]) == (E | 1u) && wall_clock64() <= deadline) __builtin_amdgcn_s_sleep(2);
+            while (agent_peek(&state[j]) == (E | 1u) && !(held = wall_clock64() > deadline))
+                __builtin_amdgcn_s_sleep(2);
         }
         __threadfence();
         group_sync();
-        if (lane_id() == 0) atomicOr(&P.slot_free[spec_slot >> 6], (unsigned long long)(1ull << (spec_slot & 63)));
+        // A helper that still holds an item after the deadline may still
+        // write into the record: the record then stays out of the free set
+        // for the rest of the launch, so that no later publication reuses it.
+        if (ballot(held)) {
+            if (status == 0) status = GCL_E_TIMEOUT;
+        } else if (lane_id() == 0) {
+            atomicOr(&P.slot_free[spec_slot >> 6], (unsigned long long)(1ull << (spec_slot & 63)));
+        }
         group_sync();
         spec_slot = -1;
     }
@@ -2187,7 +2273,17 @@ struct Worker {
         nodes = 0;
         steps = 0;
         csr_bytes = 0;
+        part_bytes = 0;
         status = 0;
+        {
+            // this search's time limit (never past the worker's own end)
+            const uint64_t now = wall_clock64();
+            const uint64_t nn = (uint64_t)n * (uint64_t)n;
+            const uint64_t per = P.graph_ticks_per_n2;
+            uint64_t lim = P.graph_ticks_base;
+            lim = (per != 0 && nn > (~0ull - lim) / per) ? ~0ull : lim + per * nn;
+            deadline = lim > life - min(life, now) ? life : now + lim;
+        }
         num_aut = 0;
         level = 0;
         seq_size = 0;
@@ -2203,7 +2299,6 @@ struct Worker {
         spec_slot = -1;
         root_changed = false;
         fix_low = 0xFFFFFFFFu;
-        spec_maxn = P.maxn;
         // counted in *pending by the host before the launch
         spec_pending = P.spec_min_n > 0 && n >= P.spec_min_n;
         init_partition(P.init_order, P.color, vbase);
@@ -2215,8 +2310,12 @@ struct Worker {
         ++nodes;
         group_sync();
         while (level >= 1 && status == 0) {
-            if (budget == 0 || wall_clock64() > deadline) {
-                status = GCL_E_INTERNAL;
+            if (budget == 0) {
+                status = GCL_E_INTERNAL;  // step budget exhausted
+                break;
+            }
+            if (wall_clock64() > deadline) {
+                status = GCL_E_TIMEOUT;
                 break;
             }
             --budget;
@@ -2259,22 +2358,36 @@ struct Worker {
         positions_from_elem();
         rebuild_from_lasts();
         const uint32_t v = ((const ix_t*)(rec + L.vert))[j];
-        const uint64_t s0 = steps, b0 = csr_bytes;
+        const uint64_t s0 = steps, b0 = csr_bytes, pb0 = part_bytes;
         budget = step_budget;
         status = 0;
         individualize_and_refine(v, tcf);
-        const uint64_t n1 = (uint64_t)P.maxn + 2, nw1 = (uint64_t)P.maxn / 64 + 2;
-        ix_t* oe = (ix_t*)(rec + L.elem) + (uint64_t)j * n1;
-        uint64_t* ol = (uint64_t*)(rec + L.lasts) + (uint64_t)j * nw1;
-        for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) oe[p] = H.elem(p);
-        for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) ol[w] = H.lasts()[w];
+        // The owner classifies a root child whose code is below the best
+        // path's at level 2 and differs from the first path's as kBad from
+        // the code alone (spec_take).  The best path's code only grows, so a
+        // child below the value read here is rejected whenever the owner
+        // takes it: its partition is not written (most root children of a
+        // graph without automorphisms are such children).
+        const int32_t f1 = uni(hd->first1);
+        const int32_t c1 = uni((int32_t)__hip_atomic_load(&hd->canon1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const bool code_only = status == 0 && code != f1 && code < c1;
+        if (!code_only) {
+            // j < count, which the publisher bounded by the record's capacity for n
+            uint8_t* item = rec + L.payload + (uint64_t)j * spec_item_bytes(n);
+            ix_t* oe = (ix_t*)item;
+            uint64_t* ol = (uint64_t*)(item + spec_item_lasts(n));
+            for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) oe[p] = H.elem(p);
+            for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) ol[w] = H.lasts()[w];
+        }
         if (lane_id() == 0) {
             // a failed refinement is handed to the owner as a code no
             // refinement produces (codes are below 2^15)
-            ((uint32_t*)(rec + L.code))[j] = status ? kFailedCode : (uint32_t)code;
+            ((uint32_t*)(rec + L.code))[j] = status ? kFailedCode : ((uint32_t)code | (code_only ? kCodeOnly : 0u));
             ((uint32_t*)(rec + L.ncells))[j] = ncells;
-            ((uint64_t*)(rec + L.steps))[2 * j] = steps - s0;
-            ((uint64_t*)(rec + L.steps))[2 * j + 1] = csr_bytes - b0;
+            uint64_t* cr = (uint64_t*)(rec + L.steps) + 4ull * j;
+            cr[0] = steps - s0;
+            cr[1] = csr_bytes - b0;
+            cr[2] = part_bytes - pb0;
         }
         __threadfence();
         group_sync();
@@ -2286,7 +2399,7 @@ struct Worker {
 // Tries to claim and refine one published root child.  Returns 1 when it did
 // some work, 0 when there was nothing to take now, -1 when helping is over.
 template <class Hot>
-__device__ int help_once(Worker<Hot>& W, const KParams& P) {
+__device__ __forceinline__ int help_once(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
     const uint32_t nwr = (P.spec_records + 63) / 64;
     const uint64_t word = (uint32_t)l < nwr
@@ -2332,7 +2445,7 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
                 const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
                 const uint32_t idx = atomicAdd(&hd->next, 1u);
-                if (idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
+                if (idx < cnt && idx < L.slots && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
                     claim = idx;
                     E = e;
                     slot_sel = slot;
@@ -2350,7 +2463,7 @@ __device__ int help_once(Worker<Hot>& W, const KParams& P) {
 
 // help_once, with its time and the children refined counted for diagnostics.
 template <class Hot>
-__device__ int help_counted(Worker<Hot>& W, const KParams& P) {
+__device__ __forceinline__ int help_counted(Worker<Hot>& W, const KParams& P) {
     const uint64_t t0 = wall_clock64();
     const int h = help_once(W, P);
     if (h > 0) {
@@ -2384,7 +2497,8 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.levels = L.levels;
     W.step_budget = P.step_budget;
     const uint64_t now = wall_clock64();
-    W.deadline = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
+    W.life = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
+    W.deadline = W.life;
     W.steps = 0;
     W.dg_nodes = W.dg_steps = W.dg_helped = W.dg_search = W.dg_wait = W.dg_help = W.dg_idle = 0;
     W.dg_refine = W.dg_classify = W.dg_orbits = W.dg_backtrack = 0;
@@ -2420,7 +2534,7 @@ __device__ void write_diag(Worker<Hot>& W, const KParams& P, uint64_t t_start) {
 // Searches graph g (vertices vtx_off[g] .. vtx_off[g+1]-1) and writes its
 // labels and result record.
 template <class Hot>
-__device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
+__device__ __forceinline__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
     const int l = lane_id();
     const int64_t vbase = P.vtx_off[g];
     W.n = (uint32_t)(P.vtx_off[g + 1] - vbase);
@@ -2444,6 +2558,7 @@ __device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
         res.automorphisms = (uint64_t)W.num_aut;
         res.splitter_steps = W.steps;
         res.csr_bytes = W.csr_bytes;
+        res.part_bytes = W.part_bytes;
         res.device_ns = t1 - t0;
         res.status = W.status;
         res.pad = 0;
@@ -2453,7 +2568,7 @@ __device__ void process_graph(Worker<Hot>& W, const KParams& P, int32_t g) {
 }
 
 template <class Hot>
-__device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
+__device__ __forceinline__ void worker_loop(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
     const uint64_t t_start = wall_clock64();
     bool graphs_left = true;
@@ -2473,7 +2588,7 @@ __device__ void worker_loop(Worker<Hot>& W, const KParams& P) {
             }
             graphs_left = false;
         }
-        if (P.spec_min_n == 0 || wall_clock64() > W.deadline) break;
+        if (P.spec_min_n == 0 || wall_clock64() > W.life) break;
         const int h = help_counted(W, P);
         if (h < 0) break;
         if (h == 0) {
@@ -2511,7 +2626,7 @@ __device__ __forceinline__ void serve_complete(const ServeParams& S, uint32_t sl
 // stops the queue (or the worker's deadline passes).  Root children of large
 // graphs published by other workers are helped first, as in worker_loop.
 template <class Hot>
-__device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
+__device__ __forceinline__ void serve_loop(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
     const ServeParams& S = P.sv;
     uint32_t idle = 1;
@@ -2614,7 +2729,7 @@ __device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
             idle = 1;
             continue;
         }
-        if (uni(readlane(stop, 0)) || wall_clock64() > W.deadline) break;
+        if (uni(readlane(stop, 0)) || wall_clock64() > W.life) break;
         // nothing to do: back off (up to ~0.9 ms).  With thousands of idle
         // workers a request or a published root child is still seen within
         // microseconds, and their polls (loads, and the lease's CAS on one
@@ -2626,9 +2741,16 @@ __device__ void serve_loop(Worker<Hot>& W, const KParams& P) {
 }
 
 // The worker object lives in LDS: its uniform fields are reloaded where used
-// instead of occupying registers for the whole persistent loop.
+// instead of occupying registers for the whole persistent loop.  The launch
+// parameters are read from device memory through a pointer (scalar loads, as
+// from the kernel-argument segment), so that a search function the compiler
+// keeps out of line can take them by reference without a private copy.  The
+// classes up to 512 vertices hold 3 waves per SIMD (at most 168 VGPRs); the
+// larger ones are bound by LDS to fewer.
 template <int MAXN>
-__global__ __launch_bounds__(kWave) void search_kernel_lds(KParams P) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MAXN <= 512 ? 3 : (MAXN <= 1024 ? 2 : 1))))
+void search_kernel_lds(const KParams* __restrict__ Pp) {
+    const KParams& P = *Pp;
     __shared__ Worker<HotL<MAXN>> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
     const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false, P.aut_pool != nullptr);
@@ -2638,7 +2760,9 @@ __global__ __launch_bounds__(kWave) void search_kernel_lds(KParams P) {
 
 // Serving kernel of the same size class: runs until the host stops its queue.
 template <int MAXN>
-__global__ __launch_bounds__(kWave) void serve_kernel_lds(KParams P) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MAXN <= 512 ? 3 : (MAXN <= 1024 ? 2 : 1))))
+void serve_kernel_lds(const KParams* __restrict__ Pp) {
+    const KParams& P = *Pp;
     __shared__ Worker<HotL<MAXN>> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
     const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false, P.aut_pool != nullptr);
@@ -2646,7 +2770,8 @@ __global__ __launch_bounds__(kWave) void serve_kernel_lds(KParams P) {
     serve_loop(W, P);
 }
 
-__global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
+__global__ __launch_bounds__(kWave) void search_kernel_global(const KParams* __restrict__ Pp) {
+    const KParams& P = *Pp;
     __shared__ Worker<HotG> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
     const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true, P.aut_pool != nullptr);
@@ -2659,7 +2784,8 @@ __global__ __launch_bounds__(kWave) void search_kernel_global(KParams P) {
     worker_loop(W, P);
 }
 
-__global__ __launch_bounds__(kWave) void serve_kernel_global(KParams P) {
+__global__ __launch_bounds__(kWave) void serve_kernel_global(const KParams* __restrict__ Pp) {
+    const KParams& P = *Pp;
     __shared__ Worker<HotG> W;
     uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
     const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true, P.aut_pool != nullptr);

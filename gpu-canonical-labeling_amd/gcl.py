@@ -24,7 +24,8 @@ CLI_PATH = os.path.join(_HERE, "gpu_canon_label")
 
 GCL_OK = 0
 _STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "device error", -4: "no device",
-           -5: "i/o error", -6: "capacity exceeded", -7: "internal error", -8: "busy"}
+           -5: "i/o error", -6: "capacity exceeded", -7: "internal error", -8: "busy",
+           -9: "time limit exceeded"}
 ENGINES = {"gpu": 0, "parallel": 0, "cpu": 1, "sequential": 1}
 
 
@@ -46,7 +47,7 @@ class _Graph(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("automorphisms", C.c_uint64), ("refinements", C.c_uint64),
                 ("splitter_steps", C.c_uint64), ("csr_bytes", C.c_uint64), ("kernel_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("alg_bytes", C.c_uint64)]
 
 
 class _Batch(C.Structure):
@@ -57,18 +58,19 @@ class _Batch(C.Structure):
 class _BatchStats(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("automorphisms", C.c_uint64), ("splitter_steps", C.c_uint64),
                 ("csr_bytes", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("workers", C.c_int32)]
+                ("workers", C.c_int32), ("pad", C.c_int32), ("alg_bytes", C.c_uint64)]
 
 
 class _ClassStats(C.Structure):
     _fields_ = [("max_n", C.c_int32), ("graphs", C.c_int32), ("workers", C.c_int32), ("pad", C.c_int32),
                 ("kernel_ms", C.c_double), ("nodes", C.c_uint64), ("splitter_steps", C.c_uint64),
-                ("csr_bytes", C.c_uint64)]
+                ("csr_bytes", C.c_uint64), ("alg_bytes", C.c_uint64)]
 
 
 class _ServerConfig(C.Structure):
     _fields_ = [("max_n", C.c_int32), ("max_graph_nnz", C.c_int64), ("max_aut", C.c_uint32), ("slots", C.c_int32),
-                ("queue_bytes", C.c_uint64), ("lifetime_ms", C.c_int32)]
+                ("queue_bytes", C.c_uint64), ("lifetime_ms", C.c_int32), ("large_share_pct", C.c_int32),
+                ("max_search_cost", C.c_uint64)]
 
 
 class _ServerResult(C.Structure):
@@ -80,7 +82,7 @@ class _ServerResult(C.Structure):
 class _ServerStats(C.Structure):
     _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "refused_busy", "launches", "graphs_launched",
                                            "pending", "in_flight", "queued_bytes")] + \
-               [("sched_publish_ms", C.c_double), ("sched_collect_ms", C.c_double)]
+               [("sched_publish_ms", C.c_double), ("sched_collect_ms", C.c_double), ("refused_capacity", C.c_uint64)]
 
 
 _lib = None
@@ -283,6 +285,7 @@ class CanonResult:
     csr_bytes: int
     kernel_ms: float
     total_ms: float
+    alg_bytes: int = 0
 
     def dense(self, n):
         d = np.zeros(n * n, np.uint8)
@@ -299,7 +302,7 @@ def canon_label(g: Graph, max_aut=10000, engine="gpu") -> CanonResult:
     _check(lib().gcl_canon_label(C.byref(c), max_aut, ENGINES[engine], _ptr(label), _ptr(keys), None,
                                  C.byref(st)))
     return CanonResult(label[:g.n], keys[:g.nnz], st.nodes, st.automorphisms, st.splitter_steps, st.csr_bytes,
-                       st.kernel_ms, st.total_ms)
+                       st.kernel_ms, st.total_ms, st.alg_bytes)
 
 
 def nauty_parallel(g: Graph, max_aut=10000):
@@ -411,6 +414,7 @@ class BatchResult:
     kernel_ms: float
     total_ms: float
     workers: int
+    alg_bytes: int = 0
 
 
 def mix_batch(count, seed, n_min=16, n_max=1024, threads=0) -> Batch:
@@ -492,7 +496,7 @@ class DeviceBatch:
         st = _BatchStats()
         _check(lib().gcl_device_batch_results(self.h, _ptr(labels), _ptr(nodes), C.byref(st)))
         return BatchResult(labels[:int(b.vtx_off[-1])], nodes[:b.num_graphs], st.nodes, st.automorphisms,
-                           st.splitter_steps, st.csr_bytes, 0.0, 0.0, st.workers)
+                           st.splitter_steps, st.csr_bytes, 0.0, 0.0, st.workers, st.alg_bytes)
 
     def graph_stats(self):
         """Per-graph (nodes, automorphisms, splitter_steps, device_ns) of the last run."""
@@ -503,8 +507,8 @@ class DeviceBatch:
 
     def class_stats(self):
         """Per size class of the last run: dicts with max_n, graphs, workers,
-        kernel_ms (HIP events on the class's stream), nodes, splitter_steps
-        and csr_bytes."""
+        kernel_ms (HIP events on the class's stream), nodes, splitter_steps,
+        csr_bytes and alg_bytes (all algorithmic bytes of refinement)."""
         cnt = C.c_int32(0)
         _check(lib().gcl_device_batch_class_stats(self.h, None, 0, C.byref(cnt)))
         arr = (_ClassStats * max(cnt.value, 1))()
@@ -534,7 +538,7 @@ def batch_canon(batch: Batch, max_aut=10000, engine="gpu", plain=False) -> Batch
         _check(lib().gcl_batch_canon(C.byref(c), max_aut, ENGINES[engine], _ptr(labels), _ptr(nodes),
                                      C.byref(st)))
     return BatchResult(labels[:int(batch.vtx_off[-1])], nodes[:batch.num_graphs], st.nodes, st.automorphisms,
-                       st.splitter_steps, st.csr_bytes, st.kernel_ms, st.total_ms, st.workers)
+                       st.splitter_steps, st.csr_bytes, st.kernel_ms, st.total_ms, st.workers, st.alg_bytes)
 
 
 def set_side_files(prefix):

@@ -33,7 +33,8 @@ typedef enum gcl_status {
     GCL_E_IO = -5,        /* file could not be read or written */
     GCL_E_CAPACITY = -6,  /* input beyond a documented size limit */
     GCL_E_INTERNAL = -7,  /* consistency check failed inside the library */
-    GCL_E_BUSY = -8       /* admission refused (server full) */
+    GCL_E_BUSY = -8,      /* admission refused (server full) */
+    GCL_E_TIMEOUT = -9    /* a search or a device wait passed its time limit */
 } gcl_status;
 
 typedef enum gcl_engine {
@@ -57,6 +58,10 @@ typedef struct gcl_stats {
     uint64_t csr_bytes;      /* algorithmic CSR bytes read by refinement */
     double kernel_ms;        /* device time of the search launch(es) */
     double total_ms;         /* wall time of the call */
+    uint64_t alg_bytes;      /* all algorithmic bytes of refinement: csr_bytes
+                                plus 8 per non-singleton cell hashed under a
+                                non-singleton splitter, 8 per vertex of a split
+                                cell and 8n per refinement call */
 } gcl_stats;
 
 /* ---------------------------------------------------------------- errors */
@@ -164,6 +169,8 @@ typedef struct gcl_batch_stats {
     double kernel_ms;
     double total_ms;
     int32_t workers;         /* concurrent search workers of the launch */
+    int32_t pad;
+    uint64_t alg_bytes;      /* as in gcl_stats */
 } gcl_batch_stats;
 
 /*
@@ -218,6 +225,7 @@ typedef struct gcl_class_stats {
     uint64_t nodes;
     uint64_t splitter_steps;
     uint64_t csr_bytes;
+    uint64_t alg_bytes;      /* as in gcl_stats */
 } gcl_class_stats;
 gcl_status gcl_device_batch_class_stats(gcl_device_batch* db, gcl_class_stats* out, int32_t cap,
                                         int32_t* count);
@@ -330,9 +338,23 @@ typedef struct gcl_server_config {
     int32_t max_n;             /* largest graph accepted; default 4096 */
     int64_t max_graph_nnz;     /* directed edges of the largest graph; default 2^20 */
     uint32_t max_aut;          /* automorphism ring per search (>= 1); default 10000 */
-    int32_t slots;             /* request slots per size class; 0 (default): twice its workers */
+    int32_t slots;             /* request slots per size class; 0 (default): twice the
+                                  workers of the first class, up to 64 in the large class */
     uint64_t queue_bytes;      /* footprint budget of queued + running requests; default 2^30 */
-    int32_t lifetime_ms;       /* serving kernels stop after this long; 0 (default): no limit */
+    int32_t lifetime_ms;       /* serving kernels stop after this long; 0 (default): no limit.
+                                  Each request's search has its own limit either way
+                                  (60 s + n^2 / 10^4 ms, or GCL_TIME_LIMIT_MS) */
+    int32_t large_share_pct;   /* with max_n > 1024: percent of the chip's worker slots for
+                                  the class above 1024 vertices (5..80); the first class
+                                  gets 90 minus this.  Default 40 */
+    uint64_t max_search_cost;  /* admission cap on a graph's estimated search cost in
+                                  splitter steps: n^2 for a graph whose vertices all have
+                                  one degree and one colour (refinement cannot split its
+                                  root; without automorphisms every one of its n root
+                                  children then refines to about n steps), n otherwise.
+                                  A graph above it is refused with GCL_E_CAPACITY.
+                                  0: no cap.  Default 1.5e8 (regular graphs up to
+                                  n = 12,247) */
 } gcl_server_config;
 
 /* Fills the defaults listed above. */
@@ -359,6 +381,8 @@ typedef struct gcl_server_stats {
     uint64_t pending, in_flight, queued_bytes;
     double sched_publish_ms;   /* scheduler thread: writing requests into slots and publishing them */
     double sched_collect_ms;   /* scheduler thread: reading results, pair verdicts */
+    uint64_t refused_capacity; /* requests refused with GCL_E_CAPACITY (a graph larger than
+                                  every class, or above max_search_cost) */
 } gcl_server_stats;
 
 gcl_status gcl_server_create(const gcl_server_config* cfg, gcl_server** out);

@@ -52,6 +52,11 @@ typedef struct {
     uint32_t* work;        /* work_cell, n */
     uint32_t* bucket;      /* n+1 */
     uint64_t steps;
+    /* algorithmic bytes of refinement: 12 per splitter vertex, 4 per
+     * neighbour id read, 8 per non-singleton cell hashed under a
+     * non-singleton splitter, 8 per vertex of a cell that splits, 8n per
+     * call (one load and one store of the element vector) */
+    uint64_t bytes;
 } oscratch;
 
 /* cell_index_by_first (partition_refinement.cpp:12-25) */
@@ -130,6 +135,7 @@ static int o_refine(opart* P, const ograph* g, const uint32_t* act_first,
     memset(active, 0, n + 1);
     memset(cnt, 0, sizeof(uint32_t) * n);
     memset(bucket, 0, sizeof(uint32_t) * (n + 1));
+    s->bytes += 8ull * n;
 
 #define ADD_ACTIVE(f)                                  \
     do {                                               \
@@ -182,8 +188,10 @@ static int o_refine(opart* P, const ograph* g, const uint32_t* act_first,
         if (W.length > 1) longcode = O_MASH(longcode, (uint64_t)W.length);
 
         /* neighbour counts of every vertex into W (244-251) */
+        s->bytes += 12ull * W.length;
         for (uint32_t i = 0; i < W.length; ++i) {
             uint32_t w = wverts[i];
+            s->bytes += 4ull * (g->rp[w + 1] - g->rp[w]);
             for (uint32_t e = g->rp[w]; e < g->rp[w + 1]; ++e) ++cnt[g->col[e]];
         }
 
@@ -192,6 +200,7 @@ static int o_refine(opart* P, const ograph* g, const uint32_t* act_first,
             int ci = i;
             ocell C = P->lcs[ci];
             if (C.length == 1) continue;
+            if (W.length > 1) s->bytes += 8;  /* the cell header read for the hash */
             uint32_t* cv = &P->elem[C.first];
             uint32_t L = C.length;
             uint32_t common = cnt[cv[0]];
@@ -208,6 +217,7 @@ static int o_refine(opart* P, const ograph* g, const uint32_t* act_first,
                     longcode = O_MASH(longcode, (uint64_t)(common + C.first));
                 continue;
             }
+            s->bytes += 8ull * L;  /* the split cell's ids, read and written */
             for (uint32_t d = bmin; d <= bmax; ++d) bucket[d] = 0;
             for (uint32_t j = 0; j < L; ++j) ++bucket[cnt[cv[j]]];
 
@@ -807,6 +817,7 @@ int oracle_canon(int32_t n_, const int32_t* row_ptr, const int32_t* col,
         stats->refinements = S.refinements;
         stats->splitter_steps = S.s.steps;
         stats->stale_classify = S.stale;
+        stats->alg_bytes = S.s.bytes;
     }
 
 done:

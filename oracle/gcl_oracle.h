@@ -24,6 +24,11 @@ typedef struct oracle_stats {
     uint64_t refinements;     /* refinement calls, root included */
     uint64_t splitter_steps;  /* splitter iterations over all refinements */
     uint64_t stale_classify;  /* classify_node calls that reused an old code */
+    uint64_t alg_bytes;       /* algorithmic bytes of all refinement calls:
+                                 12 per splitter vertex, 4 per neighbour id,
+                                 8 per non-singleton cell hashed under a
+                                 non-singleton splitter, 8 per vertex of a
+                                 split cell, 8n per call */
 } oracle_stats;
 
 /*

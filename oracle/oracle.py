@@ -15,7 +15,7 @@ REF_BIN = os.path.join(_HERE, "_ref", "gpu_canon_label_ref")
 
 class Stats(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("automorphisms", C.c_uint64), ("refinements", C.c_uint64),
-                ("splitter_steps", C.c_uint64), ("stale_classify", C.c_uint64)]
+                ("splitter_steps", C.c_uint64), ("stale_classify", C.c_uint64), ("alg_bytes", C.c_uint64)]
 
 
 _lib = None
@@ -59,7 +59,7 @@ def canon(n, row_ptr, col, color=None, max_aut=10000, want_dense=True, aut_cap=0
     if rc != 0:
         raise RuntimeError("oracle_canon failed")
     out = dict(label=label[:n], nodes=st.nodes, automorphisms=st.automorphisms, refinements=st.refinements,
-               splitter_steps=st.splitter_steps, stale_classify=st.stale_classify)
+               splitter_steps=st.splitter_steps, stale_classify=st.stale_classify, alg_bytes=st.alg_bytes)
     if want_dense:
         out["dense"] = dense.reshape(n, n)
         out["keys"] = np.nonzero(dense)[0].astype(np.uint32)

@@ -86,3 +86,39 @@ def test_batch_tuning_arguments_rejected_without_device(monkeypatch):
         raise AssertionError("expected GclError")
     except gcl.GclError as e:
         assert e.status == -1
+
+
+def test_dimacs_vertex_ids_below_one(tmp_path):
+    """'n' lines with id 0 or a negative id are ignored (the reference reads
+    the id as unsigned, so a negative one falls outside 1..n); an edge with a
+    negative id is an error."""
+    f = tmp_path / "g.dimacs"
+    f.write_text("p edge 3 2\nn 0 9\nn -5 7\nn 2 3\ne 1 2\ne 2 3\n")
+    g = gcl.read_dimacs(str(f))
+    assert g.n == 3 and list(g.color) == [0, 3, 0]
+    f.write_text("p edge 3 1\ne -1 2\n")
+    try:
+        gcl.read_dimacs(str(f))
+        raise AssertionError("expected GclError")
+    except gcl.GclError as e:
+        assert e.status == -1
+
+
+def test_empty_batch_without_arrays():
+    """A batch of zero graphs may come with null arrays: no entry point reads
+    them (on a machine without a GPU the device entry points report no
+    device)."""
+    import ctypes as C
+    b = gcl._Batch(0, None, None, None, None)
+    h = C.c_void_p()
+    st = gcl._BatchStats()
+    r = gcl.lib().gcl_device_batch_create(C.byref(b), 10, C.byref(h))
+    if gcl.device_count() == 0:
+        assert r == -4
+    else:
+        assert r == 0
+        assert gcl.lib().gcl_device_batch_run(h, 0, C.byref(st)) == 0
+        gcl.lib().gcl_device_batch_destroy(h)
+    assert gcl.lib().gcl_batch_canon(C.byref(b), 10, 1, None, None, C.byref(st)) == -1  # null labels
+    lab = (C.c_int32 * 1)()
+    assert gcl.lib().gcl_batch_canon(C.byref(b), 10, 1, lab, None, C.byref(st)) == 0

@@ -26,6 +26,8 @@ def test_gpu_golden_singles(golden):
         assert gcl.format_adjacency(r.keys, g.n).encode() == golden_output(e), e["name"]
         assert r.nodes == e["nodes"], (e["name"], r.nodes, e["nodes"])
         assert r.automorphisms == e["automorphisms"], (e["name"], r.automorphisms, e["automorphisms"])
+        o = oracle.canon_graph(g, max_aut=e["max_aut"], want_dense=False)
+        assert r.alg_bytes == o["alg_bytes"], (e["name"], r.alg_bytes, o["alg_bytes"])
 
 
 def test_gpu_golden_pairs(golden):
@@ -62,6 +64,8 @@ def test_gpu_generated_vs_oracle():
             assert np.array_equal(r.keys, o["keys"]), (kind, g.n)
             assert r.nodes == o["nodes"], (kind, g.n, r.nodes, o["nodes"])
             assert r.automorphisms == o["automorphisms"], (kind, g.n)
+            # the same traversal, so the same refinement byte count
+            assert r.alg_bytes == o["alg_bytes"], (kind, g.n, r.alg_bytes, o["alg_bytes"])
 
 
 def test_gpu_relabel_invariance():

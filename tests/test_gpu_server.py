@@ -188,3 +188,49 @@ def test_server_large_class_same_as_batch_path():
         assert got[tp]["verdict"] == want_verdict
     finally:
         srv.close()
+
+
+def test_server_large_regular_graph_and_admission_cap():
+    """The serving configuration's large 3-regular graphs: n = 10,000 (an
+    estimated 10^8 splitter steps, under the default cap of 1.5e8) is served
+    within 60 s and gets the batch path's labels; n = 13,000 (1.69e8) is
+    refused at admission with GCL_E_CAPACITY and counted."""
+    import time
+    g = gcl.permute(gcl.random_regular(10000, 3, 7), 3)
+    ref = gcl.canon_label(g, engine="gpu")  # before the server holds the GPU
+    too_big = gcl.random_regular(13000, 3, 8)
+    srv = gcl.Server(max_n=50000, max_graph_nnz=400000, lifetime_ms=180000)
+    try:
+        with pytest.raises(gcl.GclError) as ei:
+            srv.submit(too_big)
+        assert ei.value.status == -6
+        assert srv.stats()["refused_capacity"] == 1
+        _log("large regular n=10000 served")
+        t0 = time.time()
+        t = srv.submit(g)
+        res = srv.wait_all([t], timeout_s=60)[t]
+        elapsed = time.time() - t0
+        _log(f"served in {elapsed:.2f} s (batch path kernel {ref.kernel_ms:.0f} ms)")
+        assert res["status"] == 0
+        assert np.array_equal(res["labels"], ref.label) and res["nodes"] == ref.nodes
+        assert elapsed < 60
+    finally:
+        srv.close()
+
+
+def test_server_stopped_by_its_lifetime_fails_requests():
+    """Serving kernels that returned because their lifetime passed take no
+    more requests: a request submitted afterwards completes with
+    GCL_E_TIMEOUT, and later submissions are refused with it."""
+    import time
+    srv = gcl.Server(max_n=256, lifetime_ms=200)
+    try:
+        time.sleep(1.0)
+        t = srv.submit(gcl.torus(5, 6))
+        r = srv.wait_all([t], timeout_s=10)[t]
+        assert r["status"] == -9, r
+        with pytest.raises(gcl.GclError) as ei:
+            srv.submit(gcl.torus(5, 6))
+        assert ei.value.status == -9
+    finally:
+        srv.close()

@@ -114,3 +114,39 @@ def test_oracle_canonical_form_is_a_relabeling():
         assert np.array_equal(m, o["dense"])
         for s in range(4):
             assert np.array_equal(oracle.canon_graph(gcl.permute(g, s))["dense"], o["dense"])
+
+
+def test_oracle_algorithmic_bytes_hand_counted():
+    """The refinement byte count B of the oracle on two graphs small enough to
+    trace by hand: 12 per splitter vertex, 4 per neighbour id, 8 per
+    non-singleton cell hashed under a non-singleton splitter, 8 per vertex of
+    a split cell, 8n per refinement call.
+
+    Path 0-1-2 (n=3): root: splitter {0,1,2} 12*3 + 4*4, one cell hashed 8,
+    the cell splits into {0,2}{1} 8*3, then splitter {1} 12 + 4*2 leaves
+    {0,2} whole; + 8*3 = 128.  Two root children, each discrete right after
+    the individualization: 8*3 each.  Total 176, 3 nodes.
+
+    Star K_{1,3} (centre 0): root: splitter {0..3} 12*4 + 4*6, one cell 8,
+    split into {1,2,3}{0} 8*4, splitter {0} 12 + 4*3 leaves {1,2,3}; + 8*4 =
+    168.  Child 1: splitter {1} 12 + 4 splits nothing, + 32 = 48; its
+    children 2 and 3: 32 each (discrete).  Child 2 (not in the orbit of 1 under
+    the automorphism (2 3) found below 1): 48, its first child 32, which gives
+    the automorphism (1 2) and prunes child 3.  Total 168 + 48 + 32 + 32 + 48
+    + 32 = 360, 6 nodes."""
+    path = oracle.canon(3, np.array([0, 1, 3, 4]), np.array([1, 0, 2, 1]))
+    assert (path["alg_bytes"], path["nodes"]) == (176, 3)
+    star = oracle.canon(4, np.array([0, 3, 4, 5, 6]), np.array([1, 2, 3, 0, 0, 0]))
+    assert (star["alg_bytes"], star["nodes"]) == (360, 6)
+
+
+def test_host_engine_bytes_equal_oracle(golden):
+    """The host engine (--sequential, traversal-exact) counts the same
+    refinement bytes as the oracle, graph by graph."""
+    from conftest import golden_graph
+    for e in golden["singles"]:
+        g = golden_graph(e["name"])
+        o = oracle.canon_graph(g, max_aut=e["max_aut"], want_dense=False)
+        r = gcl.canon_label(g, max_aut=e["max_aut"], engine="cpu")
+        assert r.alg_bytes == o["alg_bytes"], (e["name"], r.alg_bytes, o["alg_bytes"])
+        assert r.alg_bytes > r.csr_bytes, e["name"]
