@@ -143,6 +143,34 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
     return GCL_OK;
 }
 
+gcl_status gcl_canon_label_noalloc(const gcl_csr* g, uint32_t max_aut, int32_t* out_label, uint32_t* out_keys,
+                                   gcl_stats* stats) {
+    gcl_status s = validate_nonempty(g);
+    if (s != GCL_OK) return s;
+    if ((s = check_max_aut(max_aut)) != GCL_OK) return s;
+    if (!out_label) return fail(GCL_E_INVALID, "null output labels");
+    auto t0 = std::chrono::steady_clock::now();
+    DeviceGraphResult res;
+    if ((s = noalloc_canon(*g, max_aut, out_label, &res)) != GCL_OK) return s;
+    try {
+        if (out_keys) canonical_keys(*g, out_label, out_keys);
+    } catch (const std::bad_alloc&) {
+        return fail(GCL_E_NOMEM, "host allocation failed");
+    }
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->nodes = res.nodes;
+        stats->automorphisms = res.automorphisms;
+        stats->refinements = res.nodes;
+        stats->splitter_steps = res.splitter_steps;
+        stats->csr_bytes = res.csr_bytes;
+        stats->alg_bytes = res.csr_bytes + res.part_bytes;
+        stats->kernel_ms = (double)res.device_ns / 1e6;  // device wall clock of the search
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return GCL_OK;
+}
+
 gcl_status gcl_canon_automorphisms(const gcl_csr* g, uint32_t max_aut, int32_t* out_label, int32_t* out_auts,
                                    int64_t cap_rows, int64_t* rows, gcl_stats* stats) {
     gcl_status s = validate_nonempty(g);

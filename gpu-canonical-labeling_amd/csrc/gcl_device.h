@@ -138,6 +138,11 @@ gcl_status serve_wait(ServeEngine* s);
 // returned because its lifetime passed
 gcl_status serve_check(ServeEngine* s);
 
+// Canonical labeling of one graph (n <= 1024, nnz <= 32768) with no device
+// allocation: every device buffer lives in a static device array (see
+// gcl_canon_label_noalloc in gcl.h).  out_label: n ints (host).
+gcl_status noalloc_canon(const gcl_csr& g, uint32_t max_aut, int32_t* out_label, DeviceGraphResult* res);
+
 int32_t device_count();
 uint64_t device_wall_ticks_per_ms();
 uint64_t step_budget();

@@ -983,6 +983,175 @@ void serve_stop(ServeEngine* s) { s->stop(); }
 gcl_status serve_wait(ServeEngine* s) { return s->wait(); }
 gcl_status serve_check(ServeEngine* s) { return s->check(); }
 
+// ----------------------------------------------------- allocation-free path
+// One graph searched with no device allocation: every buffer the search
+// needs beyond the caller's host input and output (launch parameters, work
+// counters, the speculation board, the staged CSR, labels and result record,
+// the owner's search scratch and one speculation record) is carved out of the
+// file-scope array `scratch`.  Each device that runs the path holds its own
+// copy of the array for the life of the process; its address is taken with
+// hipGetSymbolAddress on the first call on that device, and only that address
+// is used from the host.  Calls in one process are serialized (a host mutex):
+// each call overwrites the array, and the kernels of two overlapping calls on
+// one device would race on it.
+namespace {
+constexpr uint32_t kNoAllocMaxN = 1024;
+constexpr uint64_t kNoAllocMaxNnz = 32768;
+constexpr uint64_t kNoAllocBytes = 6ull << 20;
+constexpr uint32_t kNoAllocRing = 64;  // automorphism slots (the batch ring)
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mu;
+uint8_t* g_noalloc_ptr[64] = {};  // per device: the address of `scratch`
+struct NoAllocDev {
+    hipDeviceProp_t prop;
+    int per_cu[kClasses];
+};
+NoAllocDev g_noalloc_dev[64];
+
+// Offsets inside the array for a graph of size class maxn.
+struct NoAllocLayout {
+    uint64_t params, ctr, board, vtx, row, col, color, init, label, out, items, spec, work, total;
+    uint64_t spec_stride;
+    Layout L;
+};
+NoAllocLayout noalloc_layout(uint32_t maxn, uint64_t max_nnz) {
+    NoAllocLayout A;
+    uint64_t o = 0;
+    A.params = o; o = align256(o + sizeof(KParams));
+    A.ctr = o; o = align256(o + 64 * sizeof(unsigned int));
+    A.board = o; o = align256(o + 2 * sizeof(unsigned long long));
+    A.vtx = o; o = align256(o + 2 * sizeof(int64_t));
+    A.row = o; o = align256(o + ((uint64_t)maxn + 1) * sizeof(int64_t));
+    A.col = o; o = align256(o + std::max<uint64_t>(max_nnz, 1) * sizeof(int32_t));
+    A.color = o; o = align256(o + (uint64_t)maxn * sizeof(int32_t));
+    A.init = o; o = align256(o + (uint64_t)maxn * sizeof(int32_t));
+    A.label = o; o = align256(o + (uint64_t)maxn * sizeof(int32_t));
+    A.out = o; o = align256(o + sizeof(DeviceGraphResult));
+    A.items = o; o = align256(o + sizeof(int32_t));
+    A.spec_stride = spec_record_bytes(maxn, maxn, maxn);  // every root child
+    A.spec = o; o = align256(o + A.spec_stride);
+    const uint64_t arena = std::min<uint64_t>((uint64_t)maxn * (maxn + 1) / 2, 64ull * maxn) + 64;
+    A.L = make_layout(maxn, max_nnz, kNoAllocRing, arena, maxn + 2, false);
+    A.work = o; o = align256(o + A.L.total);
+    A.total = o;
+    return A;
+}
+}  // namespace
+
+gcl_status noalloc_canon(const gcl_csr& g, uint32_t max_aut, int32_t* out_label, DeviceGraphResult* res) {
+    const int32_t n = g.n;
+    if (n > (int32_t)kNoAllocMaxN || g.nnz > (int64_t)kNoAllocMaxNnz)
+        return fail(GCL_E_CAPACITY, "graph (n=%d, nnz=%lld) exceeds the allocation-free path's limit (n <= %u, nnz <= %llu)",
+                    n, (long long)g.nnz, kNoAllocMaxN, (unsigned long long)kNoAllocMaxNnz);
+    if (device_count() <= 0) return fail(GCL_E_NODEVICE, "no HIP device visible");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    GCL_HIP(hipSetDevice(0));
+    int dev = 0;
+    GCL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(GCL_E_DEVICE, "device %d out of range", dev);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        GCL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        NoAllocDev& d = g_noalloc_dev[dev];
+        GCL_HIP(hipGetDeviceProperties(&d.prop, dev));
+        for (int c = 0; c < kClasses; ++c) {
+            d.per_cu[c] = 0;
+            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&d.per_cu[c], class_kernel(c), kWave, 0));
+            if (d.per_cu[c] <= 0) d.per_cu[c] = 1;
+        }
+        g_noalloc_ptr[dev] = (uint8_t*)ptr;
+    }
+    uint8_t* base = g_noalloc_ptr[dev];
+    const NoAllocDev& d = g_noalloc_dev[dev];
+    int cls = 0;
+    while (cls < kClasses - 1 && (uint32_t)n > kClassN[cls]) ++cls;
+    const uint32_t maxn = kClassN[cls];
+    const NoAllocLayout A = noalloc_layout(maxn, kNoAllocMaxNnz);
+    if (A.total > kNoAllocBytes) return fail(GCL_E_INTERNAL, "allocation-free layout exceeds its array");
+    // host-side staging of the graph (the device copies go into the array)
+    std::vector<int64_t> row((size_t)n + 1);
+    for (int32_t i = 0; i <= n; ++i) row[(size_t)i] = g.row_ptr[i];
+    std::vector<int32_t> init;
+    if (g.color) {
+        init.resize((size_t)n);
+        std::iota(init.begin(), init.end(), 0);
+        std::stable_sort(init.begin(), init.end(), [&](int32_t x, int32_t y) { return g.color[x] < g.color[y]; });
+    }
+    const int64_t vtx[2] = {0, n};
+    const int32_t item0 = 0;
+    const uint32_t smin = spec_min_n();
+    const bool spec = smin > 0 && (uint32_t)n >= smin;
+    // counters: work head 0; pending (graphs that may publish) 1 or 0
+    unsigned int ctr[64] = {};
+    ctr[1] = spec ? 1u : 0u;
+    const unsigned long long board[2] = {1ull, 0ull};  // record 0 free, none published
+    const int64_t resident = (int64_t)d.per_cu[cls] * d.prop.multiProcessorCount;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(spec ? 1 + (int64_t)n : 1, resident));
+    const TimeLimits lim = time_limits((double)n * (double)n);
+    KParams P;
+    memset(&P, 0, sizeof(P));
+    P.num_items = 1;
+    P.items = (const int32_t*)(base + A.items);
+    P.vtx_off = (const int64_t*)(base + A.vtx);
+    P.row_ptr = (const int64_t*)(base + A.row);
+    P.col = (const int32_t*)(base + A.col);
+    P.init_order = g.color ? (const int32_t*)(base + A.init) : nullptr;
+    P.color = g.color ? (const int32_t*)(base + A.color) : nullptr;
+    P.out_label = (int32_t*)(base + A.label);
+    P.out = (DeviceGraphResult*)(base + A.out);
+    P.next_item = (unsigned int*)(base + A.ctr);
+    P.pending = (unsigned int*)(base + A.ctr) + 1;
+    // the launch holds one graph: only the worker that takes it searches and
+    // uses search scratch; the others only refine root children in LDS and
+    // write the speculation record, so every worker is given the same slice
+    P.scratch = base + A.work;
+    P.scratch_stride = 0;
+    P.maxn = maxn;
+    P.max_nnz = kNoAllocMaxNnz;
+    P.ring = std::max(1u, std::min(max_aut, kNoAllocRing));
+    P.max_aut = max_aut;
+    P.arena_cap = std::min<uint64_t>((uint64_t)maxn * (maxn + 1) / 2, 64ull * maxn) + 64;
+    P.levels = maxn + 2;
+    P.step_budget = step_budget();
+    P.time_limit = lim.launch;
+    P.graph_ticks_base = lim.base;
+    P.graph_ticks_per_n2 = lim.per_n2;
+    if (spec) {
+        P.spec = base + A.spec;
+        P.spec_stride = A.spec_stride;
+        P.spec_slots = maxn;
+        P.spec_records = 1;
+        P.spec_min_n = smin;
+        P.slot_free = (unsigned long long*)(base + A.board);
+        P.board_mask = (unsigned long long*)(base + A.board) + 1;
+    }
+    hipStream_t st = 0;
+    GCL_HIP(hipMemcpyAsync(base + A.params, &P, sizeof(P), hipMemcpyHostToDevice, st));
+    GCL_HIP(hipMemcpyAsync(base + A.ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, st));
+    GCL_HIP(hipMemcpyAsync(base + A.board, board, sizeof(board), hipMemcpyHostToDevice, st));
+    GCL_HIP(hipMemcpyAsync(base + A.vtx, vtx, sizeof(vtx), hipMemcpyHostToDevice, st));
+    GCL_HIP(hipMemcpyAsync(base + A.items, &item0, sizeof(item0), hipMemcpyHostToDevice, st));
+    GCL_HIP(hipMemcpyAsync(base + A.row, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, st));
+    if (g.nnz > 0)
+        GCL_HIP(hipMemcpyAsync(base + A.col, g.col, sizeof(int32_t) * (size_t)g.nnz, hipMemcpyHostToDevice, st));
+    if (g.color) {
+        GCL_HIP(hipMemcpyAsync(base + A.color, g.color, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+        GCL_HIP(hipMemcpyAsync(base + A.init, init.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    }
+    hipError_t e = launch_class(cls, (const KParams*)(base + A.params), (int)grid, st);
+    if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+    GCL_HIP(hipMemcpyAsync(out_label, base + A.label, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    DeviceGraphResult r;
+    GCL_HIP(hipMemcpyAsync(&r, base + A.out, sizeof(r), hipMemcpyDeviceToHost, st));
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(GCL_E_DEVICE, "search kernel failed: %s", hipGetErrorString(e));
+    r.device_ns = (uint64_t)((double)r.device_ns * 1e6 / (double)device_wall_ticks_per_ms());
+    if (res) *res = r;
+    if (r.status != 0) return fail((gcl_status)r.status, "device search failed (%s)", gcl_status_string((gcl_status)r.status));
+    return GCL_OK;
+}
+
 BatchTuning default_batch_tuning() {
     BatchTuning t;
     t.spec_min_n = spec_min_n();

@@ -125,6 +125,7 @@ _SIGS = {
     "gcl_device_batch_class_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gcl_device_batch_graph_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_gen_mix_batch": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "gcl_canon_label_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gcl_server_config_default": (None, [C.c_void_p]),
     "gcl_server_create": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gcl_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -436,6 +437,18 @@ def mix_batch(count, seed, n_min=16, n_max=1024, threads=0) -> Batch:
 
 class _Tuning(C.Structure):
     _fields_ = [("spec_min_n", C.c_int32), ("min_class_n", C.c_int32)]
+
+
+def canon_label_noalloc(g: Graph, max_aut=10000) -> CanonResult:
+    """gcl_canon_label_noalloc: the GPU search of one graph (n <= 1024,
+    nnz <= 32768) with no device allocation (a static device array)."""
+    c = g._csr()
+    label = np.zeros(max(g.n, 1), np.int32)
+    keys = np.zeros(max(g.nnz, 1), np.uint32)
+    st = _Stats()
+    _check(lib().gcl_canon_label_noalloc(C.byref(c), max_aut, _ptr(label), _ptr(keys), C.byref(st)))
+    return CanonResult(label[:g.n], keys[:g.nnz], st.nodes, st.automorphisms, st.splitter_steps, st.csr_bytes,
+                       st.kernel_ms, st.total_ms, st.alg_bytes)
 
 
 def canon_automorphisms(g: "Graph", max_aut=10000):

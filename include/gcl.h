@@ -123,6 +123,24 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
                            int32_t* out_label, uint32_t* out_keys,
                            uint8_t* out_dense, gcl_stats* stats);
 
+/*
+ * gcl_canon_label on the GPU with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every device buffer the search needs (launch parameters, counters, the
+ * staged CSR, labels, result record, search scratch and one speculation
+ * record) lives in a static device array of 6 MiB that each device running
+ * this call holds until the process ends; the call makes no hipMalloc,
+ * hipFree or other allocating device call.  Limits: n <= 1024 and
+ * nnz <= 32768 (GCL_E_CAPACITY beyond); automorphism ring of
+ * min(max_aut, 64) slots, as the batch entry points (labels are those of
+ * gcl_canon_label, node counts may be lower).  Calls in one process are
+ * serialized; the array keeps the last call's contents.
+ *   out_label  n ints (required), out_keys nnz uint32 or NULL, stats or NULL
+ */
+gcl_status gcl_canon_label_noalloc(const gcl_csr* g, uint32_t max_aut, int32_t* out_label,
+                                   uint32_t* out_keys, gcl_stats* stats);
+
 /* The automorphisms the GPU search stored for one graph, for callers that
  * check or reuse them.  No counterpart in the reference: its store is
  * internal (sequential.cpp:515,552; nauty_parallel.cu).  Labels as
