@@ -7,6 +7,8 @@ produces
     gpu-canonical-labeling_amd/libgcl.so        C ABI of include/gcl.h
     gpu-canonical-labeling_amd/gpu_canon_label  CLI (reference main.cpp surface)
     oracle/liboracle.so                         CPU oracle (test checker)
+    hbm_probe                                   1 GiB copy kernel that calibrates
+                                                the traffic counters
     oracle/_ref/gpu_canon_label_ref             reference sequential binary,
                                                 only where the reference
                                                 sources are present
@@ -71,8 +73,9 @@ def build(with_reference=True):
     _run_all(cmds)
     lib = os.path.join(PKG, "libgcl.so")
     _run_all([[HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", lib] + objs])
-    probe = os.path.join(PKG, "hbm_probe")
-    _run_all([[HIPCC] + HIPFLAGS + [os.path.join(CSRC, "hbm_probe.hip"), "-o", probe]])
+    # the copy kernel that calibrates the HBM traffic counters (beside bench.py)
+    probe = os.path.join(REPO, "hbm_probe")
+    _run_all([[HIPCC] + HIPFLAGS + [os.path.join(REPO, "hbm_probe.hip"), "-o", probe]])
     cli = os.path.join(PKG, "gpu_canon_label")
     _run_all([[CXX] + CXXFLAGS + [os.path.join(CSRC, "gpu_canon_label.cpp"), "-o", cli,
                                   "-L" + PKG, "-lgcl", "-Wl,-rpath,$ORIGIN"]])

@@ -466,6 +466,16 @@ struct Server {
                     blocked = true;
                     break;
                 }
+                if (r == GCL_E_CAPACITY) {  // refused at admission: that is its result
+                    gcl_server_result& o = out[next];
+                    o = gcl_server_result{};
+                    o.status = GCL_E_CAPACITY;
+                    o.verdict = -1;
+                    o.n = a.n;
+                    ++next;
+                    ++got;
+                    continue;
+                }
                 if (r != GCL_OK) return r;
                 idx[ticket] = next++;
             }

@@ -1447,7 +1447,10 @@ struct Worker {
             H.sync();
             if (e == p && deg <= (uint32_t)kWave) singleton_step(X, u, deg, longcode, hint);
             else general_step(X, p, e, longcode, hint);
-            if ((X.steps & 0x3FFFu) == 0) flush_split_bytes(X);
+            // (an LDS class has n <= 8192, so a lane's count stays below
+            // n^2 < 2^32 over a whole refinement)
+            if constexpr (!Hot::kLds)
+                if ((X.steps & 0x3FFFu) == 0) flush_split_bytes(X);
         }
         flush_split_bytes(X);
         budget = bud;

@@ -425,8 +425,9 @@ gcl_status gcl_server_stats_get(gcl_server* s, gcl_server_stats* st);
  * pair_with[i] when pair_with[i] >= 0 (NULL: all canonicalize), submitted at
  * arrival_s[i] seconds after the call starts (non-decreasing); a request
  * refused as busy is retried and its latency counts from the scheduled
- * arrival.  Blocks until every request has completed; out[i] gets the result
- * of request i.
+ * arrival; a request refused with GCL_E_CAPACITY gets that status as its
+ * result (latency 0).  Blocks until every request has completed; out[i] gets
+ * the result of request i.
  */
 gcl_status gcl_server_replay(gcl_server* s, const gcl_batch* graphs, const int32_t* pair_with,
                              const double* arrival_s, int32_t count, gcl_server_result* out);
