@@ -44,6 +44,8 @@ struct DevBuf {
 };
 
 constexpr int kClasses = 8;
+// classes of at most this many graphs get descent records (two per graph)
+constexpr uint32_t kDescGraphs = 8;
 constexpr uint32_t kClassN[kClasses] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
 
 template <int MAXN>
@@ -52,7 +54,27 @@ hipError_t launch_lds(const KParams* P, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_class(int cls, const KParams* P, int grid, hipStream_t s) {
+template <int MAXN>
+hipError_t launch_lds_desc(const KParams* P, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(search_kernel_lds_desc<MAXN>, dim3(grid), dim3(kWave), 0, s, P);
+    return hipGetLastError();
+}
+
+// Classes from 1024 vertices up have a kernel with descent speculation.
+constexpr int kDescMinClass = 4;
+
+hipError_t launch_class(int cls, bool desc, const KParams* P, int grid, hipStream_t s) {
+    if (desc && cls >= kDescMinClass) {
+        switch (cls) {
+            case 4: return launch_lds_desc<1024>(P, grid, s);
+            case 5: return launch_lds_desc<2048>(P, grid, s);
+            case 6: return launch_lds_desc<4096>(P, grid, s);
+            case 7: return launch_lds_desc<8192>(P, grid, s);
+            default:
+                hipLaunchKernelGGL(search_kernel_global_desc, dim3(grid), dim3(kWave), 0, s, P);
+                return hipGetLastError();
+        }
+    }
     switch (cls) {
         case 0: return launch_lds<64>(P, grid, s);
         case 1: return launch_lds<128>(P, grid, s);
@@ -68,7 +90,16 @@ hipError_t launch_class(int cls, const KParams* P, int grid, hipStream_t s) {
     }
 }
 
-const void* class_kernel(int cls) {
+const void* class_kernel(int cls, bool desc = false) {
+    if (desc && cls >= kDescMinClass) {
+        switch (cls) {
+            case 4: return (const void*)search_kernel_lds_desc<1024>;
+            case 5: return (const void*)search_kernel_lds_desc<2048>;
+            case 6: return (const void*)search_kernel_lds_desc<4096>;
+            case 7: return (const void*)search_kernel_lds_desc<8192>;
+            default: return (const void*)search_kernel_global_desc;
+        }
+    }
     switch (cls) {
         case 0: return (const void*)search_kernel_lds<64>;
         case 1: return (const void*)search_kernel_lds<128>;
@@ -170,8 +201,9 @@ struct DeviceBatch {
         int32_t count = 0;
         int64_t grid = 0;
         size_t item_off = 0;
+        bool desc = false;  // the class's kernel carries descent speculation
         KParams P;
-        DevBuf scratch, spec, board, diag, autpool;
+        DevBuf scratch, spec, board, diag, autpool, dspec, dboard;
         uint64_t spec_work = 0;  // root children of the class's large graphs
         uint32_t spec_big = 0;   // graphs of the class that may publish
         hipStream_t stream = nullptr;
@@ -322,8 +354,9 @@ struct DeviceBatch {
             const uint32_t levels = std::min<uint32_t>(maxn + 2, 4096u);
             const uint32_t spec_min = tune.spec_min_n;
             const bool spec = spec_min > 0 && cls_maxn[cls] >= spec_min;
+            C.desc = spec && cnt <= kDescGraphs && cls >= kDescMinClass;
             int per_cu = 0;
-            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls), kWave, 0));
+            GCL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, class_kernel(cls, C.desc), kWave, 0));
             if (per_cu <= 0) per_cu = 1;
             const int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
             // automorphism rings: only the worker that owns a graph stores
@@ -365,6 +398,30 @@ struct DeviceBatch {
                 for (uint32_t r = 0; r < spec_records; ++r) masks[r / 64] |= 1ull << (r % 64);  // all free
                 GCL_HIP(hipMalloc(&C.board.p, sizeof(unsigned long long) * masks.size()));
                 GCL_HIP(hipMemcpy(C.board.p, masks.data(), sizeof(unsigned long long) * masks.size(),
+                                  hipMemcpyHostToDevice));
+            }
+            // descent records: a class of a few graphs (single-graph and pair
+            // calls) has idle workers to spare for every graph's first path
+            uint32_t d_records = 0, d_items = 0, d_depth = 0, d_path = 0;
+            uint64_t d_stride = 0;
+            if (C.desc) {
+                // two per graph: the current best path's and the one retired
+                // at the last new best leaf
+                d_records = 2 * (uint32_t)cnt;
+                d_depth = std::min<uint32_t>(512u, maxn);
+                d_path = std::min<uint32_t>(levels - 1, 2048u);
+                d_items = 1024;
+                const DescLayout D = make_desc_layout(maxn, d_items, d_depth, d_path);
+                const uint64_t budget = 4ull << 30;
+                if (desc_record_bytes(D) > budget)
+                    d_items = (uint32_t)std::max<uint64_t>(64, (budget - D.payload) / D.item_bytes);
+                d_stride = align256(desc_record_bytes(make_desc_layout(maxn, d_items, d_depth, d_path)));
+                GCL_HIP(hipMalloc(&C.dspec.p, (size_t)(d_stride * d_records)));
+                GCL_HIP(hipMemset(C.dspec.p, 0, (size_t)(d_stride * d_records)));
+                std::vector<unsigned long long> masks(2, 0ull);
+                for (uint32_t r = 0; r < d_records; ++r) masks[0] |= 1ull << r;  // all free, none published
+                GCL_HIP(hipMalloc(&C.dboard.p, sizeof(unsigned long long) * masks.size()));
+                GCL_HIP(hipMemcpy(C.dboard.p, masks.data(), sizeof(unsigned long long) * masks.size(),
                                   hipMemcpyHostToDevice));
             }
             GCL_HIP(hipMalloc(&C.scratch.p, (size_t)(L.total * (uint64_t)C.grid)));
@@ -414,6 +471,16 @@ struct DeviceBatch {
             P.aut_pool = pool_rings ? (uint8_t*)C.autpool.p : nullptr;
             P.aut_stride = pool_rings ? ring_bytes : 0;
             C.aut_stride = P.aut_stride;
+            if (d_records > 0) {
+                P.dspec = (uint8_t*)C.dspec.p;
+                P.dspec_stride = d_stride;
+                P.dspec_records = d_records;
+                P.dspec_items = d_items;
+                P.dspec_depth = d_depth;
+                P.dspec_path = d_path;
+                P.dslot_free = (unsigned long long*)C.dboard.p;
+                P.dboard = (unsigned long long*)C.dboard.p + 1;
+            }
             workers += (int32_t)C.grid;
             item_off += cnt;
             classes.push_back(std::move(C));
@@ -452,7 +519,7 @@ struct DeviceBatch {
             const KParams* dP = (const KParams*)params.p + (it.base() - 1 - classes.begin());
             if ((e = hipStreamWaitEvent(c.stream, start, 0)) != hipSuccess) return e;
             if (timed && (e = hipEventRecord(c.t0, c.stream)) != hipSuccess) return e;
-            if ((e = launch_class(c.cls, dP, (int)c.grid, c.stream)) != hipSuccess) return e;
+            if ((e = launch_class(c.cls, c.desc, dP, (int)c.grid, c.stream)) != hipSuccess) return e;
             if (timed && (e = hipEventRecord(c.t1, c.stream)) != hipSuccess) return e;
             if ((e = hipEventRecord(c.done, c.stream)) != hipSuccess) return e;
         }
@@ -1139,7 +1206,7 @@ gcl_status noalloc_canon(const gcl_csr& g, uint32_t max_aut, int32_t* out_label,
         GCL_HIP(hipMemcpyAsync(base + A.color, g.color, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
         GCL_HIP(hipMemcpyAsync(base + A.init, init.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
     }
-    hipError_t e = launch_class(cls, (const KParams*)(base + A.params), (int)grid, st);
+    hipError_t e = launch_class(cls, false, (const KParams*)(base + A.params), (int)grid, st);
     if (e != hipSuccess) return fail(GCL_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
     GCL_HIP(hipMemcpyAsync(out_label, base + A.label, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
     DeviceGraphResult r;

@@ -164,6 +164,70 @@ __host__ __device__ inline uint64_t spec_record_bytes(uint32_t maxn, uint32_t sl
     return align256(make_spec_layout(maxn, slots).payload + spec_item_bytes(maxn) * (uint64_t)items);
 }
 
+// Descent speculation (single large graphs).  After its first leaf, a worker
+// publishes a descent record: for each level L >= 2 of the first path, the
+// members of the level's target cell after the first-path vertex (up to
+// kDescPerLevel of them), deepest level first.  A helper that claims item
+// (L, v) rebuilds the first path's partition at level L from the record,
+// individualizes v and follows the leftmost path below it (the smallest
+// member of each level's target cell, which is always its orbit's
+// representative), recording per level what the owner's search would
+// compute there; the owner replays such a descent when its exact sequential
+// traversal reaches (L, v) on the first path.
+constexpr uint32_t kDescPerLevel = 8;    // items per first-path level
+constexpr uint32_t kDescMembers = 16;    // target cells larger than this end a recorded descent
+struct DescHeader {
+    uint32_t epoch, graph, count, next, n, levels;  // levels: first-path snapshots copied
+    uint32_t pad[2];
+};
+// One recorded level of a descent.
+struct DescLevel {
+    uint32_t vert, code, ncells;  // vertex individualized, refinement code, cells after it
+    uint32_t tc_first, tc_len;    // this level's target cell (tc_len 0: none recorded)
+    uint32_t pad;
+    uint64_t steps, csr, part;    // splitter steps and algorithmic bytes of the refinement
+    uint16_t mem[kDescMembers];   // target cell members in increasing id
+};
+struct DescLayout {
+    uint64_t header, elem, snaps, item_level, item_vert, state, nrec, level_first, level_count, payload;
+    uint64_t item_bytes, lev_bytes, lasts_off, elem_off;
+    uint32_t items, depth, path;
+};
+// maxn: the launch's largest n; items, depth: item capacity and levels
+// recorded per item; path: first-path levels whose snapshots the record holds
+__host__ __device__ inline DescLayout make_desc_layout(uint32_t maxn, uint32_t items, uint32_t depth, uint32_t path) {
+    DescLayout L;
+    const uint64_t n1 = (uint64_t)maxn + 2;
+    const uint64_t nw1 = (uint64_t)maxn / 64 + 2;
+    uint64_t o = 0;
+    L.items = items;
+    L.depth = depth;
+    L.path = path;
+    L.header = o; o = align256(o + sizeof(DescHeader));
+    L.elem = o; o = align256(o + 2 * n1);
+    L.snaps = o; o = align256(o + 8 * nw1 * ((uint64_t)path + 1));
+    L.item_level = o; o = align256(o + 2 * (uint64_t)items);
+    L.item_vert = o; o = align256(o + 2 * (uint64_t)items);
+    L.state = o; o = align256(o + 4 * (uint64_t)items);
+    L.nrec = o; o = align256(o + 4 * (uint64_t)items);
+    L.level_first = o; o = align256(o + 4 * ((uint64_t)path + 1));
+    L.level_count = o; o = align256(o + 4 * ((uint64_t)path + 1));
+    L.payload = o;
+    // per item: `depth` DescLevel records, then `depth` bitmaps, then the
+    // element vector of the last recorded level
+    L.lev_bytes = sizeof(DescLevel);
+    L.lasts_off = align256(L.lev_bytes * depth);
+    L.elem_off = L.lasts_off + align256(8 * nw1 * depth);
+    L.item_bytes = L.elem_off + align256(2 * n1);
+    return L;
+}
+__host__ __device__ inline uint64_t desc_record_bytes(const DescLayout& L) {
+    return L.payload + L.item_bytes * (uint64_t)L.items;
+}
+// nrec word of an item: levels recorded, and these flags
+constexpr uint32_t kDescLeaf = 1u << 30;    // the last recorded level is a leaf
+constexpr uint32_t kDescFailed = 1u << 31;  // the helper's refinement failed
+
 // Serving mode: requests published by the host while the kernel runs.  The
 // host side (fine-grained host memory, read with system-scope loads) holds
 // the queue and each request slot's graph; the claiming worker copies the
@@ -232,6 +296,13 @@ struct KParams {
     DiagRecord* diag;           // per-worker records (grid entries), or NULL
     uint8_t* aut_pool;          // automorphism rings, one per item (aut_stride bytes), or NULL
     uint64_t aut_stride;
+    // descent speculation: records, per-record layout sizes, free set and
+    // board (as for the root records); dspec == nullptr disables it
+    uint8_t* dspec;
+    uint64_t dspec_stride;
+    uint32_t dspec_records, dspec_items, dspec_depth, dspec_path;
+    unsigned long long* dslot_free;
+    unsigned long long* dboard;
     ServeParams sv;             // serving mode when sv.tail != nullptr
 };
 
@@ -381,6 +452,7 @@ struct LdsHot {
 template <int MAXN>
 struct HotL {
     static constexpr bool kLds = true;
+    static constexpr bool kDesc = false;
     static constexpr bool kRowLds = MAXN < 1024;
     __device__ __forceinline__ static LdsHot<MAXN>& S() {
         __shared__ LdsHot<MAXN> s;
@@ -430,6 +502,7 @@ struct LdsBits {
 // the bitmaps stay in LDS.
 struct HotG {
     static constexpr bool kLds = false;
+    static constexpr bool kDesc = false;
     ix_t* h16;
     uint32_t* h32;
     uint64_t* h64;
@@ -468,6 +541,14 @@ struct HotG {
         nw1 = (uint64_t)maxn / 64 + 2;
         (void)base;
     }
+};
+
+// A hot-state type with descent speculation compiled in.  Only the kernels
+// of the classes where single large graphs run carry it, so that the
+// kernels of the batch's many small graphs keep their register budget.
+template <class Base>
+struct WithDesc : Base {
+    static constexpr bool kDesc = true;
 };
 
 // Register copies of the worker fields that every refinement step touches
@@ -534,6 +615,14 @@ struct Worker {
     }
     uint32_t dg_graphs;
     int32_t spec_slot;       // board slot published by this worker, or -1
+    // descent record published by this worker for its current best path
+    // (-1: none), and the record it retired when a new best leaf was found
+    // (-1: none), whose claimed items may still be in progress
+    int32_t d_slot, d_old;
+    uint32_t d_epoch, d_old_epoch;
+    bool d_renew;  // a new best leaf: publish its path's descents
+    uint8_t* d_rec;
+    DescLayout DLy;
     bool spec_pending;       // counted in *pending until it publishes or ends
     uint32_t spec_epoch, spec_cursor, spec_count;
     uint8_t* spec_rec;
@@ -1562,31 +1651,44 @@ struct Worker {
         truncate_fix();
         const uint32_t ss = seq_size;
         uint32_t nsub = 0;
-        for (int64_t base = from; base < upto; base += kWave) {
-            const int64_t i = base + l;
-            bool ok = false;
-            if (i < upto) {
-                uint32_t x = aut_fix[i];
-                if ((x >> 1) >= ss) {
-                    ok = true;
-                } else if (!(x & 1u)) {
-                    const ix_t* a = auts + (uint64_t)i * n;
-                    ok = true;
-                    for (uint32_t j = x >> 1; j < ss; ++j) {
-                        const uint32_t sj = seq[j];
-                        if (a[sj] != sj) {
-                            ok = false;
-                            x = (j << 1) | 1u;
-                            break;
-                        }
-                    }
-                    if (ok) x = ss << 1;
-                    aut_fix[i] = x;
-                }
+        // the fix words of up to 8 * 64 slots are loaded together (one
+        // memory latency per chunk instead of one per 64 slots); most slots
+        // are settled by their word alone
+        constexpr int kChunk = 8;
+        for (int64_t base = from; base < upto; base += kWave * kChunk) {
+            uint32_t xs[kChunk];
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) {
+                const int64_t i = base + c * kWave + l;
+                xs[c] = i < upto ? aut_fix[i] : 0u;
             }
-            const uint64_t m = ballot(ok);
-            if (ok) subgroup[nsub + lanes_below(m)] = (uint32_t)i;
-            nsub += (uint32_t)__popcll(m);
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) {
+                const int64_t i = base + c * kWave + l;
+                bool ok = false;
+                if (i < upto) {
+                    uint32_t x = xs[c];
+                    if ((x >> 1) >= ss) {
+                        ok = true;
+                    } else if (!(x & 1u)) {
+                        const ix_t* a = auts + (uint64_t)i * n;
+                        ok = true;
+                        for (uint32_t j = x >> 1; j < ss; ++j) {
+                            const uint32_t sj = seq[j];
+                            if (a[sj] != sj) {
+                                ok = false;
+                                x = (j << 1) | 1u;
+                                break;
+                            }
+                        }
+                        if (ok) x = ss << 1;
+                        aut_fix[i] = x;
+                    }
+                }
+                const uint64_t m = ballot(ok);
+                if (ok) subgroup[nsub + lanes_below(m)] = (uint32_t)i;
+                nsub += (uint32_t)__popcll(m);
+            }
         }
         group_sync();
         if (nsub == 0) return;
@@ -1890,10 +1992,12 @@ struct Worker {
         snapshot(level);
     }
 
-    __device__ bool first_path_node() {
+    __device__ bool first_path_node(const KParams& P, uint32_t graph) {
         // first_path_node (reference src/sequential.cpp:216-257)
         if (ncells == n) {
             first_terminal();
+            if constexpr (Hot::kDesc)
+                if (P.dspec) desc_publish(P, graph, level);
             backtrack(level - 1);
             return true;
         }
@@ -1958,6 +2062,7 @@ struct Worker {
                 store_cert(best_keys, best_row, best_label);
                 for (uint32_t i = (uint32_t)lane_id(); i < seq_size; i += kWave) best_seq[i] = seq[i];
                 best_seq_size = seq_size;
+                if constexpr (Hot::kDesc) d_renew = true;
                 group_sync();
                 backtrack(L - 1);
                 break;
@@ -2205,6 +2310,420 @@ struct Worker {
         spec_slot = -1;
     }
 
+    // ---------------------------------------------- descent speculation
+    __device__ DescHeader* desc_header() { return (DescHeader*)(d_rec + DLy.header); }
+    __device__ uint32_t* desc_state() { return (uint32_t*)(d_rec + DLy.state); }
+
+    // At a new best leaf of level d (the first leaf is the first of them):
+    // lv[0 .. d-2] are the best path's target cells, each cursor just after
+    // the path's vertex, snap[1 .. d] the path's boundary snapshots and
+    // H.elem the leaf's element order (backtracking since has only rebuilt
+    // cells).  Publishes items (L, v) for L = 2 .. d-1, v the members of
+    // level L's target cell after the path's vertex, deepest level first.
+    __device__ void desc_publish(const KParams& P, uint32_t graph, uint32_t leaf_level) {
+        d_slot = -1;
+        if (P.spec_min_n == 0 || n < P.spec_min_n || leaf_level < 3) return;
+        const int l = lane_id();
+        const uint32_t lmax = min(leaf_level - 1, P.dspec_path);
+        if (lmax < 2) return;
+        // take a free record
+        int32_t slot = -1;
+        if (l == 0) {
+            const uint32_t nwr = (P.dspec_records + 63) / 64;
+            for (uint32_t w = 0; w < nwr && slot < 0; ++w) {
+                uint64_t m = __hip_atomic_load(&P.dslot_free[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (m) {
+                    const uint64_t bit = m & (~m + 1);
+                    const uint64_t old = atomicAnd(&P.dslot_free[w], ~(unsigned long long)bit);
+                    if (old & bit) {
+                        slot = (int32_t)(w * 64u + (uint32_t)__builtin_ctzll(bit));
+                        break;
+                    }
+                    m = old & ~bit;
+                }
+            }
+        }
+        slot = __shfl(slot, 0, kWave);
+        if (slot < 0) return;
+        d_rec = P.dspec + (uint64_t)slot * P.dspec_stride;
+        DLy = make_desc_layout(P.maxn, P.dspec_items, P.dspec_depth, P.dspec_path);
+        const uint64_t nw1 = (uint64_t)P.maxn / 64 + 2;
+        uint8_t* rec = d_rec;
+        uint32_t* lfirst = (uint32_t*)(rec + DLy.level_first);
+        uint32_t* lcount = (uint32_t*)(rec + DLy.level_count);
+        ix_t* ilev = (ix_t*)(rec + DLy.item_level);
+        ix_t* ivert = (ix_t*)(rec + DLy.item_vert);
+        // items, deepest level first: lane q of a chunk takes level lmax - q
+        uint32_t count = 0;
+        for (uint32_t base = 0; base + 2 <= lmax; base += kWave) {
+            const uint32_t q = base + (uint32_t)l;
+            const bool valid = q + 2 <= lmax;
+            const uint32_t L = valid ? lmax - q : 0;
+            uint32_t c = 0;
+            LevelRec r;
+            if (valid) {
+                r = lv[L - 1];
+                c = r.cursor < r.tc_len ? min(r.tc_len - r.cursor, kDescPerLevel) : 0u;
+            }
+            const uint32_t inc = wave_scan(c);
+            const uint32_t off = count + inc - c;
+            const uint32_t take = valid ? (off >= DLy.items ? 0u : min(c, DLy.items - off)) : 0u;
+            if (valid) {
+                lfirst[L] = off;
+                lcount[L] = take;
+                for (uint32_t i = 0; i < take; ++i) {
+                    ilev[off + i] = (ix_t)L;
+                    ivert[off + i] = arena_mem[r.arena_off + r.cursor + i];
+                }
+            }
+            count = min(count + readlane(inc, kWave - 1), DLy.items);
+        }
+        count = uni(count);
+        if (count == 0) {
+            if (l == 0) atomicOr(&P.dslot_free[slot >> 6], (unsigned long long)(1ull << (slot & 63)));
+            group_sync();
+            return;
+        }
+        // levels 0 and 1 have no items
+        if (l < 2) {
+            lfirst[l] = 0;
+            lcount[l] = 0;
+        }
+        // the first leaf's element vector and the first path's snapshots
+        ix_t* re = (ix_t*)(rec + DLy.elem);
+        for (uint32_t p = (uint32_t)l; p < n; p += kWave) re[p] = H.elem(p);
+        uint64_t* rs = (uint64_t*)(rec + DLy.snaps);
+        for (uint32_t L = 2; L <= lmax; ++L)
+            for (uint32_t w = (uint32_t)l; w < nw; w += kWave) rs[(uint64_t)L * nw1 + w] = snap[(uint64_t)L * (nw + 1) + w];
+        DescHeader* hd = desc_header();
+        const uint32_t epoch = hd->epoch + 1;
+        d_epoch = epoch;
+        if (l == 0) {
+            hd->graph = graph;
+            hd->count = count;
+            hd->n = n;
+            hd->levels = lmax;
+            __hip_atomic_store(&hd->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+        group_sync();
+        uint32_t* state = desc_state();
+        for (uint32_t j = (uint32_t)l; j < count; j += kWave) state[j] = epoch << 2;
+        __threadfence();
+        group_sync();
+        if (l == 0) {
+            agent_store(&hd->epoch, epoch);
+            __threadfence();
+            atomicOr(&P.dboard[slot >> 6], (unsigned long long)(1ull << (slot & 63)));
+        }
+        d_slot = slot;
+        group_sync();
+    }
+
+    // Takes record `slot` off the board and withdraws its unclaimed items;
+    // with `wait`, also waits for the claimed ones (up to the deadline) and
+    // returns the record to the free set (a record a helper still holds at
+    // the deadline stays out of it for the rest of the launch).
+    __device__ void desc_end(const KParams& P, int32_t slot, uint32_t epoch, bool wait) {
+        uint8_t* rec = P.dspec + (uint64_t)slot * P.dspec_stride;
+        uint32_t* state = (uint32_t*)(rec + DLy.state);
+        const uint32_t E = epoch << 2;
+        const uint32_t count = uni(((const DescHeader*)(rec + DLy.header))->count);
+        if (lane_id() == 0) atomicAnd(&P.dboard[slot >> 6], ~(unsigned long long)(1ull << (slot & 63)));
+        group_sync();
+        bool held = false;
+        for (uint32_t j = (uint32_t)lane_id(); j < count; j += kWave) {
+            agent_cas(&state[j], E | 0u, E | 3u);
+            if (wait)
+                while (agent_peek(&state[j]) == (E | 1u) && !(held = wall_clock64() > deadline))
+                    __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        group_sync();
+        if (!wait) return;
+        if (ballot(held)) {
+            if (status == 0) status = GCL_E_TIMEOUT;
+        } else if (lane_id() == 0) {
+            atomicOr(&P.dslot_free[slot >> 6], (unsigned long long)(1ull << (slot & 63)));
+        }
+        group_sync();
+    }
+    // Ends both publications (the current path's and the retired one).
+    __device__ void desc_finish(const KParams& P) {
+        if (d_slot >= 0) desc_end(P, d_slot, d_epoch, true);
+        if (d_old >= 0) desc_end(P, d_old, d_old_epoch, true);
+        d_slot = d_old = -1;
+    }
+    // A new best leaf: the current record is retired without waiting for
+    // its claimed descents (the record retired before, long since claimed,
+    // is ended first), and the best path's descents are published.
+    __device__ void desc_republish(const KParams& P, uint32_t graph) {
+        if (d_old >= 0) desc_end(P, d_old, d_old_epoch, true);
+        d_old = -1;
+        if (d_slot >= 0) {
+            desc_end(P, d_slot, d_epoch, false);
+            d_old = d_slot;
+            d_old_epoch = d_epoch;
+            d_slot = -1;
+        }
+        desc_publish(P, graph, (uint32_t)canon_level);
+    }
+
+    // The owner at level L (its node is the first path's, as the caller
+    // checks with the sequence) is about to descend into v.  Returns kTaken
+    // when a helper's recorded descent was replayed: the levels it reached
+    // were classified as the search would classify them, and the search
+    // continues from the state the last classification left.
+    __device__ int desc_take(const KParams& P, uint32_t v) {
+        const uint32_t L = level;
+        if (L < 2 || L > DLy.path || L > uni(desc_header()->levels)) return kNotTaken;
+        // on the best path, whose descents the record holds: seq[0 .. L-2]
+        // equals best_seq[0 .. L-2]
+        if (L - 1 > best_seq_size) return kNotTaken;
+        for (uint32_t base = 0; base < L - 1; base += kWave) {
+            const uint32_t i = base + (uint32_t)lane_id();
+            if (ballot(i < L - 1 && seq[i] != best_seq[i])) return kNotTaken;
+        }
+        const uint32_t lf = uni(((const uint32_t*)(d_rec + DLy.level_first))[L]);
+        const uint32_t lc = uni(((const uint32_t*)(d_rec + DLy.level_count))[L]);
+        const ix_t* ivert = (const ix_t*)(d_rec + DLy.item_vert);
+        uint32_t j = 0xFFFFFFFFu;
+        for (uint32_t i = 0; i < lc; ++i)
+            if ((uint32_t)ivert[lf + i] == v) {
+                j = lf + i;
+                break;
+            }
+        j = uni(j);
+        if (j == 0xFFFFFFFFu) return kNotTaken;
+        uint32_t* state = desc_state();
+        const uint32_t E = d_epoch << 2;
+        uint32_t won = 0;
+        if (lane_id() == 0) won = agent_cas(&state[j], E | 0u, E | 3u) ? 1u : 0u;
+        if (uni(readlane(won, 0))) return kNotTaken;  // withdrawn: descend here
+        const uint64_t tw = wall_clock64();
+        while (agent_peek(&state[j]) != (E | 2u)) {
+            if (wall_clock64() > deadline) {
+                status = GCL_E_TIMEOUT;  // a helper did not finish in time
+                return kNotTaken;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        dg_wait += wall_clock64() - tw;
+        agent_acquire();
+        const uint32_t nr = uni(((const uint32_t*)(d_rec + DLy.nrec))[j]);
+        const uint32_t nrec = nr & 0xFFFFu;
+        if ((nr & kDescFailed) || nrec == 0) return kNotTaken;
+        ++dg_adopted;
+        desc_replay(P, d_rec + DLy.payload + (uint64_t)j * DLy.item_bytes, nrec);
+        return kTaken;
+    }
+
+    // Replays a recorded descent from level L: per level what descend(),
+    // classify() and the next other_path_node() would do, with the recorded
+    // refinement results in place of refining.
+    __device__ void desc_replay(const KParams& P, const uint8_t* item, uint32_t nrec) {
+        const int l = lane_id();
+        const DescLevel* R = (const DescLevel*)item;
+        const uint64_t* LS = (const uint64_t*)(item + DLy.lasts_off);
+        const ix_t* FE = (const ix_t*)(item + DLy.elem_off);
+        const uint64_t nw1 = (uint64_t)P.maxn / 64 + 2;
+        const uint32_t L0 = level;
+        // the element vector of the last recorded level refines every level
+        // of the descent and level L0: cells come from the snapshots
+        for (uint32_t p = (uint32_t)l; p < n; p += kWave) H.elem(p) = FE[p];
+        H.sync();
+        positions_from_elem();
+        for (uint32_t k = 0; k < nrec; ++k) {
+            const uint32_t vk = uni(R[k].vert);
+            const uint64_t sk = uni(R[k].steps);
+            // descend(vk), the refinement taken from the record
+            ++nodes;
+            if (l == 0) seq[seq_size] = (ix_t)vk;
+            ++seq_size;
+            code = (int)uni(R[k].code);
+            ncells = uni(R[k].ncells);
+            steps += sk;
+            csr_bytes += uni(R[k].csr);
+            part_bytes += uni(R[k].part);
+            if (budget <= sk) {
+                status = GCL_E_INTERNAL;  // step budget exhausted
+                return;
+            }
+            budget -= sk;
+            ++level;
+            if (level >= levels) {
+                status = GCL_E_CAPACITY;
+                return;
+            }
+            uint64_t* dst = snap + (uint64_t)level * (nw + 1);
+            for (uint32_t w = (uint32_t)l; w < nw; w += kWave) dst[w] = LS[(uint64_t)k * nw1 + w];
+            if (k + 1 == nrec) {
+                for (uint32_t w = (uint32_t)l; w < nw; w += kWave) H.lasts()[w] = LS[(uint64_t)k * nw1 + w];
+                H.sync();
+                rebuild_from_lasts();
+            }
+            group_sync();
+            const uint64_t tc0 = dg_t0();
+            classify();
+            dg_add(dg_classify, tc0);
+            // backtracked (a leaf, or pruned), failed, or the last recorded level
+            if (status != 0 || level != L0 + k + 1 || k + 1 == nrec) return;
+            // the run loop's next iteration, and other_path_node at this
+            // level: it is a non-leaf not pruned, so it selects its target
+            if (budget == 0) {
+                status = GCL_E_INTERNAL;
+                return;
+            }
+            --budget;
+            if (wall_clock64() > deadline) {
+                status = GCL_E_TIMEOUT;
+                return;
+            }
+            const uint32_t tf = uni(R[k].tc_first), tl = uni(R[k].tc_len);
+            const uint32_t idx = tc_level_reached;
+            if (tl == 0 || idx + 1 != level || uni(R[k + 1].vert) != (uint32_t)R[k].mem[0]) {
+                status = GCL_E_INTERNAL;  // not the record of a leftmost descent
+                return;
+            }
+            const uint64_t off = idx == 0 ? 0 : (uint64_t)lv[idx - 1].arena_off + lv[idx - 1].tc_len;
+            if (off + tl > arena_cap) {
+                status = GCL_E_CAPACITY;
+                return;
+            }
+            for (uint32_t i = (uint32_t)l; i < tl; i += kWave) {
+                arena_mem[off + i] = R[k].mem[i];
+                arena_rep[off + i] = R[k].mem[i];
+            }
+            LevelRec r;
+            r.tc_first = tf;
+            r.tc_len = tl;
+            r.cursor = 0;
+            r.arena_off = (uint32_t)off;
+            r.last_num_aut = 0;
+            lv[idx] = r;
+            ++tc_level_reached;
+            group_sync();
+            if (num_aut > 0) {
+                const uint64_t to0 = dg_t0();
+                compute_orbits(idx);
+                dg_add(dg_orbits, to0);
+                lv[idx].last_num_aut = num_aut;
+            }
+            // the smallest member is its orbit's representative
+            lv[idx].cursor = 1;
+            group_sync();
+        }
+    }
+
+    // Records the leftmost descent of item j of a published descent record.
+    __device__ void help_desc(const KParams& P, uint8_t* rec, uint32_t j, uint32_t E) {
+        const int l = lane_id();
+        const DescLayout L = make_desc_layout(P.maxn, P.dspec_items, P.dspec_depth, P.dspec_path);
+        const DescHeader* hd = (const DescHeader*)(rec + L.header);
+        const uint32_t graph = hd->graph;
+        const int64_t vbase = P.vtx_off[graph];
+        n = (uint32_t)(P.vtx_off[graph + 1] - vbase);
+        nw = (n + 63) / 64;
+        rowp = P.row_ptr + vbase;
+        col = P.col + rowp[0];
+        H.bind(rowp, n, nullptr, P.maxn);
+        const uint64_t nw1 = (uint64_t)P.maxn / 64 + 2;
+        const uint32_t lev = ((const ix_t*)(rec + L.item_level))[j];
+        const uint32_t v = ((const ix_t*)(rec + L.item_vert))[j];
+        const ix_t* re = (const ix_t*)(rec + L.elem);
+        const uint64_t* rs = (const uint64_t*)(rec + L.snaps) + (uint64_t)lev * nw1;
+        for (uint32_t p = (uint32_t)l; p < n; p += kWave) {
+            H.elem(p) = re[p];
+            H.cnt_st(p, 0);
+            H.hits_st(p, 0);
+            H.c1_st(p, 0);
+            H.c2_st(p, 0);
+        }
+        for (uint32_t w = (uint32_t)l; w < nw; w += kWave) {
+            H.lasts()[w] = rs[w];
+            H.active()[w] = 0;
+            H.vmask()[w] = 0;
+            H.split()[w] = 0;
+            H.oldend()[w] = 0;
+        }
+        nactive = 0;
+        H.sync();
+        positions_from_elem();
+        rebuild_from_lasts();
+        uint8_t* item = rec + L.payload + (uint64_t)j * L.item_bytes;
+        DescLevel* R = (DescLevel*)item;
+        uint64_t* LS = (uint64_t*)(item + L.lasts_off);
+        ix_t* FE = (ix_t*)(item + L.elem_off);
+        budget = step_budget;
+        status = 0;
+        // level lev's target cell: the first non-singleton cell
+        int32_t f = first_set_from(H.nsmask(), 0, nw);
+        uint32_t k = 0, flags = 0, vk = v;
+        while (f >= 0) {
+            const uint64_t s0 = steps, b0 = csr_bytes, p0 = part_bytes;
+            individualize_and_refine(vk, (uint32_t)f);
+            if (status != 0) {
+                flags = kDescFailed;
+                break;
+            }
+            if (l == 0) {
+                DescLevel& d = R[k];
+                d.vert = vk;
+                d.code = (uint32_t)code;
+                d.ncells = ncells;
+                d.tc_first = 0;
+                d.tc_len = 0;
+                d.pad = 0;
+                d.steps = steps - s0;
+                d.csr = csr_bytes - b0;
+                d.part = part_bytes - p0;
+            }
+            for (uint32_t w = (uint32_t)l; w < nw; w += kWave) LS[(uint64_t)k * nw1 + w] = H.lasts()[w];
+            ++k;
+            if (ncells == n) {
+                flags = kDescLeaf;
+                break;
+            }
+            if (k == L.depth) break;
+            // the next level's target cell and its members in increasing id
+            const int32_t f2 = first_set_from(H.nsmask(), 0, nw);
+            if (f2 < 0) break;
+            const uint32_t c2 = H.cend((uint32_t)f2);
+            const uint32_t t = uni(c2) - (uint32_t)f2 + 1;
+            if (t > kDescMembers) break;
+            if ((uint32_t)l < t) set_bit_atomic(H.vmask(), H.elem((uint32_t)f2 + (uint32_t)l));
+            H.sync();
+            const int32_t m0 = first_set_from(H.vmask(), 0, nw);  // the smallest member
+            uint32_t run = 0;
+            for (uint32_t base = 0; base < nw; base += kWave) {
+                const uint32_t w = base + (uint32_t)l;
+                uint64_t bits = w < nw ? H.vmask()[w] : 0ull;
+                const uint32_t c0 = (uint32_t)__popcll(bits);
+                const uint32_t inc = wave_scan(c0);
+                uint32_t q = run + inc - c0;
+                while (bits) {
+                    R[k - 1].mem[q++] = (uint16_t)(w * 64u + (uint32_t)__builtin_ctzll(bits));
+                    bits &= bits - 1;
+                }
+                if (w < nw) H.vmask()[w] = 0;
+                run += readlane(inc, kWave - 1);
+            }
+            if (l == 0) {
+                R[k - 1].tc_first = (uint32_t)f2;
+                R[k - 1].tc_len = t;
+            }
+            H.sync();
+            vk = uni((uint32_t)m0);
+            f = f2;
+        }
+        for (uint32_t p = (uint32_t)l; p < n; p += kWave) FE[p] = H.elem(p);
+        if (l == 0) ((uint32_t*)(rec + L.nrec))[j] = k | flags;
+        __threadfence();
+        group_sync();
+        if (l == 0) agent_store(&((uint32_t*)(rec + L.state))[j], E | 2u);
+        group_sync();
+    }
+
     __device__ bool other_path_node(const KParams& P, uint32_t graph) {
         // other_path_node (reference src/sequential.cpp:304-392)
         if (ncells < n && (eqlevfirst == level || comp_canon >= 0)) {
@@ -2244,6 +2763,9 @@ struct Worker {
                 return true;
             }
             r.cursor = cur;
+            // the first path's descents are all behind once the root is reached again
+            if constexpr (Hot::kDesc)
+                if (level == 1 && (d_slot >= 0 || d_old >= 0)) desc_finish(P);
             if (level == 1 && spec_pending) spec_publish(P, r, graph);
             const uint32_t v = arena_mem[r.arena_off + cur];
             lv[li].cursor = cur + 1;
@@ -2263,6 +2785,10 @@ struct Worker {
                 group_sync();
                 snapshot(level);
             } else {
+                // a descent a helper recorded from this first-path node is
+                // replayed (and classified level by level) instead
+                if constexpr (Hot::kDesc)
+                    if (level >= 2 && d_slot >= 0 && desc_take(P, v) == kTaken) return true;
                 descend(v, r.tc_first);
             }
         }
@@ -2300,6 +2826,8 @@ struct Worker {
         first_discovered = false;
         budget = step_budget;
         spec_slot = -1;
+        d_slot = d_old = -1;
+        d_renew = false;
         root_changed = false;
         fix_low = 0xFFFFFFFFu;
         // counted in *pending by the host before the launch
@@ -2322,9 +2850,16 @@ struct Worker {
                 break;
             }
             --budget;
-            const bool ok = first_discovered ? other_path_node(P, graph) : first_path_node();
+            const bool ok = first_discovered ? other_path_node(P, graph) : first_path_node(P, graph);
             if (!ok) break;
+            if constexpr (Hot::kDesc) {
+                if (d_renew) {
+                    d_renew = false;
+                    if (P.dspec && status == 0 && level > 1) desc_republish(P, graph);
+                }
+            }
         }
+        if constexpr (Hot::kDesc) desc_finish(P);
         spec_finish(P);
     }
 
@@ -2399,11 +2934,58 @@ struct Worker {
     }
 };
 
-// Tries to claim and refine one published root child.  Returns 1 when it did
-// some work, 0 when there was nothing to take now, -1 when helping is over.
+// Tries to claim and record one published descent.  Returns 1 when it did,
+// 0 when descents are published but none could be claimed now, -1 when none
+// is published.
+template <class Hot>
+__device__ __forceinline__ int help_desc_once(Worker<Hot>& W, const KParams& P) {
+    uint32_t claim = 0xFFFFFFFFu, E = 0, slot_sel = 0, seen = 0;
+    if (lane_id() == 0) {
+        const DescLayout L = make_desc_layout(P.maxn, P.dspec_items, P.dspec_depth, P.dspec_path);
+        const uint32_t nwr = (P.dspec_records + 63) / 64;
+        for (uint32_t w = 0; w < nwr && claim == 0xFFFFFFFFu; ++w) {
+            const uint64_t bits = __hip_atomic_load(&P.dboard[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            seen |= bits != 0ull;
+            for (uint64_t b = bits; b && claim == 0xFFFFFFFFu; b &= b - 1) {
+                const uint32_t slot = w * 64u + (uint32_t)__builtin_ctzll(b);
+                uint8_t* rec = P.dspec + (uint64_t)slot * P.dspec_stride;
+                DescHeader* hd = (DescHeader*)(rec + L.header);
+                // the claim below carries the epoch, which validates these reads
+                const uint32_t e = agent_peek(&hd->epoch) << 2;
+                const uint32_t cnt = __hip_atomic_load(&hd->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__hip_atomic_load(&hd->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
+                const uint32_t idx = atomicAdd(&hd->next, 1u);
+                if (idx < cnt && idx < L.items && agent_cas(&((uint32_t*)(rec + L.state))[idx], e | 0u, e | 1u)) {
+                    claim = idx;
+                    E = e;
+                    slot_sel = slot;
+                }
+            }
+        }
+    }
+    claim = uni(readlane(claim, 0));
+    seen = uni(readlane(seen, 0));
+    if (claim == 0xFFFFFFFFu) return seen ? 0 : -1;
+    E = uni(readlane(E, 0));
+    slot_sel = uni(readlane(slot_sel, 0));
+    W.help_desc(P, P.dspec + (uint64_t)slot_sel * P.dspec_stride, claim, E);
+    return 1;
+}
+
+// Tries to claim and refine one published descent or root child.  Returns 1
+// when it did some work, 0 when there was nothing to take now, -1 when
+// helping is over.
 template <class Hot>
 __device__ __forceinline__ int help_once(Worker<Hot>& W, const KParams& P) {
     const int l = lane_id();
+    // descents first: the owner of a first path waits for them deepest first
+    int dres = -1;
+    if constexpr (Hot::kDesc) {
+        if (P.dspec) {
+            dres = help_desc_once(W, P);
+            if (dres > 0) return 1;
+        }
+    }
     const uint32_t nwr = (P.spec_records + 63) / 64;
     const uint64_t word = (uint32_t)l < nwr
                               ? __hip_atomic_load(&P.board_mask[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
@@ -2421,7 +3003,7 @@ __device__ __forceinline__ int help_once(Worker<Hot>& W, const KParams& P) {
         }
         pend = __shfl(pend, 0, kWave);
         again = __shfl(again, 0, kWave);
-        return pend == 0 && !again ? -1 : 0;
+        return pend == 0 && !again && dres < 0 ? -1 : 0;
     }
     const SpecLayout L = make_spec_layout(P.maxn, P.spec_slots);
     // spread helpers: start at a word and bit that depend on the worker
@@ -2750,15 +3332,33 @@ __device__ __forceinline__ void serve_loop(Worker<Hot>& W, const KParams& P) {
 // keeps out of line can take them by reference without a private copy.  The
 // classes up to 512 vertices hold 3 waves per SIMD (at most 168 VGPRs); the
 // larger ones are bound by LDS to fewer.
+template <class Hot>
+__device__ __forceinline__ void search_body(const KParams& P) {
+    __shared__ Worker<Hot> W;
+    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
+    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, !Hot::kLds, P.aut_pool != nullptr);
+    bind_scratch(W, P, base, L);
+    if constexpr (!Hot::kLds) {
+        W.H.h16 = (ix_t*)(base + L.hot16);
+        W.H.h32 = (uint32_t*)(base + L.hot32);
+        W.H.h64 = (uint64_t*)(base + L.hot64);
+        W.H.n1 = hot_stride(P.maxn);
+        W.H.nw1 = (uint64_t)P.maxn / 64 + 2;
+    }
+    worker_loop(W, P);
+}
+
 template <int MAXN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MAXN <= 512 ? 3 : (MAXN <= 1024 ? 2 : 1))))
 void search_kernel_lds(const KParams* __restrict__ Pp) {
-    const KParams& P = *Pp;
-    __shared__ Worker<HotL<MAXN>> W;
-    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, false, P.aut_pool != nullptr);
-    bind_scratch(W, P, base, L);
-    worker_loop(W, P);
+    search_body<HotL<MAXN>>(*Pp);
+}
+
+// The same with descent speculation (launches of a few large graphs).
+template <int MAXN>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MAXN <= 1024 ? 2 : 1)))
+void search_kernel_lds_desc(const KParams* __restrict__ Pp) {
+    search_body<WithDesc<HotL<MAXN>>>(*Pp);
 }
 
 // Serving kernel of the same size class: runs until the host stops its queue.
@@ -2774,17 +3374,11 @@ void serve_kernel_lds(const KParams* __restrict__ Pp) {
 }
 
 __global__ __launch_bounds__(kWave) void search_kernel_global(const KParams* __restrict__ Pp) {
-    const KParams& P = *Pp;
-    __shared__ Worker<HotG> W;
-    uint8_t* base = P.scratch + (uint64_t)blockIdx.x * P.scratch_stride;
-    const Layout L = make_layout(P.maxn, P.max_nnz, P.ring, P.arena_cap, P.levels, true, P.aut_pool != nullptr);
-    bind_scratch(W, P, base, L);
-    W.H.h16 = (ix_t*)(base + L.hot16);
-    W.H.h32 = (uint32_t*)(base + L.hot32);
-    W.H.h64 = (uint64_t*)(base + L.hot64);
-    W.H.n1 = hot_stride(P.maxn);
-    W.H.nw1 = (uint64_t)P.maxn / 64 + 2;
-    worker_loop(W, P);
+    search_body<HotG>(*Pp);
+}
+
+__global__ __launch_bounds__(kWave) void search_kernel_global_desc(const KParams* __restrict__ Pp) {
+    search_body<WithDesc<HotG>>(*Pp);
 }
 
 __global__ __launch_bounds__(kWave) void serve_kernel_global(const KParams* __restrict__ Pp) {
