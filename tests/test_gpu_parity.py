@@ -147,9 +147,10 @@ def test_gpu_nauty_parallel_dense():
 
 
 def test_gpu_speculation_keeps_traversal_exact(monkeypatch):
-    # root children refined by idle workers must not change the traversal:
-    # same canonical form, node and automorphism counts as the oracle, with
-    # speculation forced on for small graphs and switched off
+    # root children refined by idle workers, and descents below the best path
+    # recorded by them (single graphs of n > 512), must not change the
+    # traversal: same canonical form, node and automorphism counts as the
+    # oracle, with speculation forced on for small graphs and switched off
     rs = np.random.RandomState(7)
     er393 = None
     for s in range(12):
@@ -158,7 +159,8 @@ def test_gpu_speculation_keeps_traversal_exact(monkeypatch):
         if n == 393:
             er393 = g
     graphs = [er393, gcl.random_regular(300, 3, 21), gcl.torus(12, 12), gcl.cfi(20, True, 5),
-              gcl.erdos_renyi_mean_degree(150, 2.0, 3)]
+              gcl.erdos_renyi_mean_degree(150, 2.0, 3), gcl.cfi(64, False, 9), gcl.cfi(70, True, 4),
+              gcl.torus(30, 27)]
     for spec in ("64", "0"):
         monkeypatch.setenv("GCL_SPEC_MIN_N", spec)
         for g in graphs:
