@@ -57,7 +57,7 @@ struct Layout {
     uint32_t levels;  // snapshot levels available
 };
 
-constexpr int kHot16 = 7;  // elem pos cend cfirst tmp(=sb) tv(=c1) tcl(=c2)
+constexpr int kHot16 = 6;  // elem pos ce tmp(=sb) tv(=c1) tcl(=c2)
 constexpr int kHot32 = 2;  // cnt hits
 constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
 // element stride of the global hot arrays: even, so that the 16-bit arrays
@@ -433,7 +433,8 @@ struct LdsHot {
     static constexpr int NROW = MAXN < 1024 ? MAXN + 1 : 1;
     // tmp is also sb: the staging area of a large-cell regroup and the list
     // of non-singleton cells are used one after the other
-    ix_t elem[MAXN], pos[MAXN], cend[MAXN], cfirst[MAXN], tmp[MAXN];
+    // ce: the cell array (cell_end / cell_first merged, see Worker::cend)
+    ix_t elem[MAXN], pos[MAXN], ce[MAXN], tmp[MAXN];
     // 16-bit counters packed in pairs (atomics on the containing word)
     uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1];
     // touched-vertex / touched-cell lists of a general step; the same words
@@ -460,8 +461,7 @@ struct HotL {
     }
     __device__ __forceinline__ ix_t& elem(uint32_t i) { return S().elem[i]; }
     __device__ __forceinline__ ix_t& pos(uint32_t i) { return S().pos[i]; }
-    __device__ __forceinline__ ix_t& cend(uint32_t i) { return S().cend[i]; }
-    __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return S().cfirst[i]; }
+    __device__ __forceinline__ ix_t& ce(uint32_t i) { return S().ce[i]; }
     __device__ __forceinline__ ix_t& tmp(uint32_t i) { return S().tmp[i]; }
     __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().tmp[i]; }
     __device__ __forceinline__ ix_t& tv(uint32_t i) { return ((ix_t*)S().tvw)[i]; }
@@ -514,16 +514,15 @@ struct HotG {
     }
     __device__ __forceinline__ ix_t& elem(uint32_t i) { return h16[i]; }
     __device__ __forceinline__ ix_t& pos(uint32_t i) { return h16[n1 + i]; }
-    __device__ __forceinline__ ix_t& cend(uint32_t i) { return h16[2 * n1 + i]; }
-    __device__ __forceinline__ ix_t& cfirst(uint32_t i) { return h16[3 * n1 + i]; }
-    __device__ __forceinline__ ix_t& tmp(uint32_t i) { return h16[4 * n1 + i]; }
-    __device__ __forceinline__ ix_t& sb(uint32_t i) { return h16[4 * n1 + i]; }
-    __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[5 * n1 + i]; }
-    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[6 * n1 + i]; }
+    __device__ __forceinline__ ix_t& ce(uint32_t i) { return h16[2 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tmp(uint32_t i) { return h16[3 * n1 + i]; }
+    __device__ __forceinline__ ix_t& sb(uint32_t i) { return h16[3 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tv(uint32_t i) { return h16[4 * n1 + i]; }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[5 * n1 + i]; }
     GCL_PACKED16(cnt, h32)
     GCL_PACKED16(hits, h32 + (n1 / 2 + 1))
-    GCL_PACKED16(c1, (uint32_t*)(h16 + 5 * n1))
-    GCL_PACKED16(c2, (uint32_t*)(h16 + 6 * n1))
+    GCL_PACKED16(c1, (uint32_t*)(h16 + 4 * n1))
+    GCL_PACKED16(c2, (uint32_t*)(h16 + 5 * n1))
     __device__ __forceinline__ uint64_t* split() { return h64; }
     __device__ __forceinline__ uint64_t* oldend() { return h64 + nw1; }
     __device__ __forceinline__ uint64_t* active() { return B().active; }
@@ -628,6 +627,18 @@ struct Worker {
     uint8_t* spec_rec;
     SpecLayout SL;
 
+    // ---------------------------------------------------------- cell array
+    // One 16-bit entry per position, ce[p]: the last position of p's cell
+    // when p is not that last position, and the cell's first position at its
+    // last position (a cell is named by its last position).  Every entry of
+    // a cell other than its last is above p, the last one at most p, so
+    //   cell_end(p)   = max(ce[p], p)
+    //   cell_first(c) = ce[c]          (c a last position)
+    // A fragment [a, b] is written as ce[p] = (p == b ? a : b).
+    __device__ __forceinline__ uint32_t cend(uint32_t p) { return max((uint32_t)H.ce(p), p); }
+    __device__ __forceinline__ uint32_t cfirst(uint32_t c) { return H.ce(c); }
+    __device__ __forceinline__ static uint32_t ce_val(uint32_t p, uint32_t a, uint32_t b) { return p == b ? a : b; }
+
     // --------------------------------------------------------- bit helpers
     __device__ __forceinline__ static bool test_bit(const uint64_t* m, uint32_t p) {
         return (m[p >> 6] >> (p & 63)) & 1ull;
@@ -723,8 +734,9 @@ struct Worker {
                     const uint32_t w2 = H.tmp(w + 1);  // the last position is always a cell last
                     last = w2 * 64u + (uint32_t)__builtin_ctzll(lasts[w2]);
                 }
-                H.cend(p) = (ix_t)last;
-                if (p == 0 || test_bit(lasts, p - 1)) H.cfirst(last) = (ix_t)p;
+                // the first position of a cell writes its last entry
+                if (p != last) H.ce(p) = (ix_t)last;
+                if (p == 0 || test_bit(lasts, p - 1)) H.ce(last) = (ix_t)p;
             }
         }
         for (uint32_t base = 0; base < nw; base += kWave) {
@@ -803,8 +815,8 @@ struct Worker {
         uint32_t c = 0, f = 0, r = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
-            c = H.cend(H.pos(u));
-            f = H.cfirst(c);
+            c = cend(H.pos(u));
+            f = cfirst(c);
             ns = f != c;
         }
         if (ns) {
@@ -842,7 +854,7 @@ struct Worker {
         H.sync();
         const bool owner = ns && r == 0;
         const bool own = owner && split;
-        if (split) H.cend(f + r) = (ix_t)(f + k - 1);
+        if (split) H.ce(f + r) = (ix_t)ce_val(f + r, f, f + k - 1);  // the front fragment [f, f+k-1]
         uint32_t hkey = kNoKey;
         if (owner) {
             H.hits_st(c, 0);
@@ -850,8 +862,7 @@ struct Worker {
             H.c2_st(c, 0);
         }
         if (own) {
-            H.cfirst(f + k - 1) = (ix_t)f;
-            H.cfirst(c) = (ix_t)(f + k);
+            H.ce(c) = (ix_t)(f + k);  // the back fragment [f+k, c]
             // active-set rule (414-435): one new active cell per split
             uint32_t cand = 0xFFFFu;
             if (test_bit(H.active(), f) || k >= L - k) {
@@ -950,9 +961,11 @@ struct Worker {
                 if ((uint32_t)t + 1 == L || (kk[t + 1 < (int)N ? t + 1 : t] >> 16) != key) end = (uint32_t)t;
                 H.elem(f + (uint32_t)t) = (ix_t)v;
                 H.pos(v) = (ix_t)(f + (uint32_t)t);
-                H.cend(f + (uint32_t)t) = (ix_t)(f + end);
+                // (t descends: the fragment's first position, reached last,
+                // overwrites the entry of its last one)
+                H.ce(f + (uint32_t)t) = (ix_t)(f + end);
                 if (t == 0 || (kk[t > 0 ? t - 1 : 0] >> 16) != key) {
-                    H.cfirst(f + end) = (ix_t)(f + (uint32_t)t);
+                    H.ce(f + end) = (ix_t)(f + (uint32_t)t);
                     if (f + end != c) set_bit_atomic(H.lasts(), f + end);
                     ++nfrag;
                 }
@@ -984,7 +997,7 @@ struct Worker {
             uint32_t u = 0;
             if (i < ntv) {
                 u = H.tv(i);
-                in = H.cend(H.pos(u)) == c;
+                in = cend(H.pos(u)) == c;
             }
             const uint64_t m = ballot(in);
             const uint32_t at = tc + lanes_below(m);
@@ -1047,16 +1060,12 @@ struct Worker {
         if (act) {
             H.elem(newpos) = (ix_t)v;
             H.pos(v) = (ix_t)newpos;
-            H.cend(newpos) = (ix_t)gend;
-            if (newpos == gstart) {
-                H.cfirst(gend) = (ix_t)gstart;
-                if (gend != c) set_bit_atomic(H.lasts(), gend);
-            }
+            H.ce(newpos) = (ix_t)ce_val(newpos, gstart, gend);
+            if (newpos == gstart && gend != c) set_bit_atomic(H.lasts(), gend);
         }
         // the untouched block [f, B - 1]
-        for (uint32_t pp = f + (uint32_t)l; pp < B; pp += kWave) H.cend(pp) = (ix_t)(B - 1);
+        for (uint32_t pp = f + (uint32_t)l; pp < B; pp += kWave) H.ce(pp) = (ix_t)ce_val(pp, f, B - 1);
         if (l == 0) {
-            H.cfirst(B - 1) = (ix_t)f;
             set_bit_atomic(H.lasts(), B - 1);
             set_bit_atomic(H.split(), f);
             set_bit_atomic(H.oldend(), c);
@@ -1095,7 +1104,7 @@ struct Worker {
                 if (sp) {
                     nv = 0;
                     for (uint32_t q = f;;) {
-                        const uint32_t last = H.cend(q);
+                        const uint32_t last = cend(q);
                         ++nv;
                         if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
                         q = last + 1;
@@ -1110,7 +1119,7 @@ struct Worker {
                 const bool was_active = test_bit(H.active(), f);
                 uint32_t max_size = 0, max_first = f;
                 for (uint32_t q = f;;) {
-                    const uint32_t last = H.cend(q);
+                    const uint32_t last = cend(q);
                     const uint32_t size = last - q + 1;
                     H.hits_st(k++, H.cnt_ld(H.elem(q)) + q);
                     if (size > max_size) {
@@ -1246,8 +1255,8 @@ struct Worker {
             uint32_t c = 0;
             if (i < ntv) {
                 const uint32_t u = H.tv(i);
-                c = H.cend(H.pos(u));
-                if (H.cfirst(c) != c) firstc = H.hits_add(c) == 0u;
+                c = cend(H.pos(u));
+                if (cfirst(c) != c) firstc = H.hits_add(c) == 0u;
             }
             const uint64_t cm = ballot(firstc);
             if (firstc) H.tcl(ntc + lanes_below(cm)) = (ix_t)c;
@@ -1263,7 +1272,7 @@ struct Worker {
             uint32_t c = 0, f = 0;
             if (i < ntc) {
                 c = H.tcl(i);
-                f = H.cfirst(c);
+                f = cfirst(c);
                 H.hits_st(c, 0);
                 small = c - f + 1 <= kSmallCell;
                 big = !small;
@@ -1287,7 +1296,7 @@ struct Worker {
         // counting sort below (indices 0..R-1, cleared after each cell)
         for (uint32_t j = 0; j < nbig; ++j) {
             const uint32_t c = uni((uint32_t)H.tcl(j));
-            const uint32_t f = uni((uint32_t)H.cfirst(c));
+            const uint32_t f = uni(cfirst(c));
             const uint32_t L = c - f + 1;
             if (s > 1 && sparse_split(X, f, c, L, ntv)) {
                 X.part_bytes += 8ull * L;
@@ -1350,11 +1359,8 @@ struct Worker {
                     const uint32_t b = kv - kmin;
                     const uint32_t first = f + (b == 0 ? 0u : H.hits_ld(b - 1));
                     const uint32_t last = f + H.hits_ld(b) - 1;
-                    H.cend(P) = (ix_t)last;
-                    if (P == first) {
-                        H.cfirst(last) = (ix_t)first;
-                        if (last != c) set_bit_atomic(H.lasts(), last);
-                    }
+                    H.ce(P) = (ix_t)ce_val(P, first, last);
+                    if (P == first && last != c) set_bit_atomic(H.lasts(), last);
                 }
             }
             X.ncells += nfrag - 1;
@@ -1399,8 +1405,8 @@ struct Worker {
                 }
                 for (uint64_t m = ballot(sp); m; m &= m - 1) {
                     const uint32_t fc = readlane(f, __builtin_ctzll(m));
-                    const uint32_t K = uni((uint32_t)H.cend(fc)) - fc + 1;
-                    const uint32_t LL = uni((uint32_t)H.cend(fc + K)) - fc + 1;
+                    const uint32_t K = uni(cend(fc)) - fc + 1;
+                    const uint32_t LL = uni(cend(fc + K)) - fc + 1;
                     longcode = GCL_MASH(longcode, fc + K - 1);
                     if (uni(test_bit(H.active(), fc)) || K >= LL - K) {
                         add_active(X, fc + K);
@@ -1423,7 +1429,7 @@ struct Worker {
                 const uint32_t f = H.sb(i);
                 if (test_bit(H.split(), f)) {
                     for (uint32_t q = f;;) {
-                        const uint32_t last = H.cend(q);
+                        const uint32_t last = cend(q);
                         if (last > q) set_bit_atomic(H.nsmask(), q);
                         else clear_bit_atomic(H.nsmask(), q);
                         if (test_bit(H.oldend(), last)) {
@@ -1480,7 +1486,7 @@ struct Worker {
             uint32_t total = 0;
             for (uint32_t base = 0; base < X.n; base += kWave) {
                 const uint32_t p = base + (uint32_t)l;
-                const bool first = p < X.n && H.cfirst(H.cend(p)) == p;
+                const bool first = p < X.n && cfirst(cend(p)) == p;
                 const uint64_t b = ballot(first);
                 if (l == 0) H.active()[base >> 6] = b;
                 total += (uint32_t)__popcll(b);
@@ -1520,7 +1526,7 @@ struct Worker {
             ++X.steps;
             // the cell end and the splitter's vertex (used when the cell is a
             // singleton) are independent loads
-            const uint32_t e = uni((uint32_t)H.cend(p));
+            const uint32_t e = uni(cend(p));
             const uint32_t wp = uni((uint32_t)H.elem(p));
             // a singleton splitter's neighbour ids are requested before the
             // sync, so that the load overlaps it and the bookkeeping
@@ -1559,7 +1565,7 @@ struct Worker {
 
     // split_by_and_refine (reference src/partition_refinement.cpp:486-539)
     __device__ void individualize_and_refine(uint32_t v, uint32_t f) {
-        const uint32_t c = H.cend(f);
+        const uint32_t c = cend(f);
         const uint32_t q = H.pos(v);
         const uint32_t x = H.elem(f);
         H.sync();
@@ -1568,9 +1574,8 @@ struct Worker {
             H.pos(v) = (ix_t)f;
             H.elem(q) = (ix_t)x;
             H.pos(x) = (ix_t)q;
-            H.cend(f) = (ix_t)f;
-            H.cfirst(f) = (ix_t)f;
-            H.cfirst(c) = (ix_t)(f + 1);
+            H.ce(f) = (ix_t)f;          // {v}
+            H.ce(c) = (ix_t)(f + 1);    // the rest [f+1, c]
         }
         set_bit_u(H.nsmask(), f, false);
         set_bit_u(H.nsmask(), f + 1, c > f + 1);
@@ -1598,7 +1603,7 @@ struct Worker {
         const int32_t f = first_set_from(H.nsmask(), 0, nw);
         if (f < 0) return true;
         const int l = lane_id();
-        const uint32_t c = H.cend((uint32_t)f);
+        const uint32_t c = cend((uint32_t)f);
         const uint32_t t = c - (uint32_t)f + 1;
         const uint32_t idx = tc_level_reached;
         const uint64_t off = idx == 0 ? 0 : (uint64_t)lv[idx - 1].arena_off + lv[idx - 1].tc_len;
@@ -2688,7 +2693,7 @@ struct Worker {
             // the next level's target cell and its members in increasing id
             const int32_t f2 = first_set_from(H.nsmask(), 0, nw);
             if (f2 < 0) break;
-            const uint32_t c2 = H.cend((uint32_t)f2);
+            const uint32_t c2 = cend((uint32_t)f2);
             const uint32_t t = uni(c2) - (uint32_t)f2 + 1;
             if (t > kDescMembers) break;
             if ((uint32_t)l < t) set_bit_atomic(H.vmask(), H.elem((uint32_t)f2 + (uint32_t)l));
