@@ -1064,7 +1064,7 @@ gcl_status serve_check(ServeEngine* s) { return s->check(); }
 namespace {
 constexpr uint32_t kNoAllocMaxN = 1024;
 constexpr uint64_t kNoAllocMaxNnz = 32768;
-constexpr uint64_t kNoAllocBytes = 6ull << 20;
+constexpr uint64_t kNoAllocBytes = 8ull << 20;
 constexpr uint32_t kNoAllocRing = 64;  // automorphism slots (the batch ring)
 static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
@@ -1078,7 +1078,7 @@ NoAllocDev g_noalloc_dev[64];
 
 // Offsets inside the array for a graph of size class maxn.
 struct NoAllocLayout {
-    uint64_t params, ctr, board, vtx, row, col, color, init, label, out, items, spec, work, total;
+    uint64_t params, ctr, board, vtx, row, col, color, init, label, out, items, spec, work, lists, total;
     uint64_t spec_stride;
     Layout L;
 };
@@ -1101,6 +1101,9 @@ NoAllocLayout noalloc_layout(uint32_t maxn, uint64_t max_nnz) {
     const uint64_t arena = std::min<uint64_t>((uint64_t)maxn * (maxn + 1) / 2, 64ull * maxn) + 64;
     A.L = make_layout(maxn, max_nnz, kNoAllocRing, arena, maxn + 2, false);
     A.work = o; o = align256(o + A.L.total);
+    // every worker shares the one search slice (only the owner searches),
+    // but helpers refine root children with touched lists of their own
+    A.lists = o; if (maxn >= kListsGlobalN) o += lists_bytes(maxn) * ((uint64_t)kNoAllocMaxN + 1);
     A.total = o;
     return A;
 }
@@ -1174,6 +1177,10 @@ gcl_status noalloc_canon(const gcl_csr& g, uint32_t max_aut, int32_t* out_label,
     // write the speculation record, so every worker is given the same slice
     P.scratch = base + A.work;
     P.scratch_stride = 0;
+    if (maxn >= kListsGlobalN) {
+        P.lists = base + A.lists;
+        P.lists_stride = lists_bytes(maxn);
+    }
     P.maxn = maxn;
     P.max_nnz = kNoAllocMaxNnz;
     P.ring = std::max(1u, std::min(max_aut, kNoAllocRing));

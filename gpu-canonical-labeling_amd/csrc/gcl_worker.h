@@ -53,13 +53,20 @@ __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~u
 // Per-worker global scratch.
 struct Layout {
     uint64_t snap, seq, first_seq, best_seq, lv, arena_mem, arena_rep, firstcode, canon_code, first_label,
-        best_label, first_keys, best_keys, first_row, best_row, subgroup, auts, hot16, hot32, hot64, total;
+        best_label, first_keys, best_keys, first_row, best_row, subgroup, auts, hot16, hot32, hot64, lists, total;
     uint32_t levels;  // snapshot levels available
 };
 
 constexpr int kHot16 = 6;  // elem pos ce tmp(=sb) tv(=c1) tcl(=c2)
 constexpr int kHot32 = 2;  // cnt hits
 constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
+// LDS classes from this size keep the touched-vertex and touched-cell lists
+// of a general step in global scratch (the step keeps them in its lanes when
+// it has at most a wave of (member, neighbour) pairs), which leaves LDS room
+// for one more worker per CU.
+constexpr uint32_t kListsGlobalN = 1024;
+// the touched-vertex and touched-cell lists of one worker (maxn + 2 entries each)
+__host__ __device__ inline uint64_t lists_bytes(uint32_t maxn) { return align256(2ull * 2ull * ((uint64_t)maxn + 2)); }
 // element stride of the global hot arrays: even, so that the 16-bit arrays
 // that double as packed counters start on a 32-bit word
 __host__ __device__ inline uint64_t hot_stride(uint32_t maxn) { return ((uint64_t)maxn + 2 + 63) & ~uint64_t(63); }
@@ -100,6 +107,7 @@ __host__ __device__ inline Layout make_layout(uint32_t maxn, uint64_t max_nnz, u
     L.hot16 = o; if (hot_global) o = align256(o + 2ull * kHot16 * hot_stride(maxn));
     L.hot32 = o; if (hot_global) o = align256(o + 4ull * kHot32 * (hot_stride(maxn) / 2 + 1));
     L.hot64 = o; if (hot_global) o = align256(o + 8ull * kHot64 * nw1);
+    L.lists = o; if (!hot_global && maxn >= kListsGlobalN) o = align256(o + lists_bytes(maxn));
     L.total = o;
     return L;
 }
@@ -272,6 +280,10 @@ struct KParams {
     unsigned int* next_item;
     uint8_t* scratch;
     uint64_t scratch_stride;
+    // per-worker touched lists (kListsGlobalN classes) outside the workers'
+    // scratch, lists_stride bytes each; NULL: in each worker's scratch
+    uint8_t* lists;
+    uint64_t lists_stride;
     uint32_t maxn;
     uint64_t max_nnz;
     uint32_t ring;
@@ -431,6 +443,9 @@ template <int MAXN>
 struct LdsHot {
     static constexpr int NW = MAXN / 64 + 2;
     static constexpr int NROW = MAXN < 1024 ? MAXN + 1 : 1;
+    static constexpr bool kListG = (uint32_t)MAXN >= kListsGlobalN;
+    static constexpr int NLW = kListG ? 1 : MAXN / 2 + 1;
+    static constexpr int NCW = kListG ? MAXN / 2 + 1 : 1;
     // tmp is also sb: the staging area of a large-cell regroup and the list
     // of non-singleton cells are used one after the other
     // ce: the cell array (cell_end / cell_first merged, see Worker::cend)
@@ -439,14 +454,19 @@ struct LdsHot {
     uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1];
     // touched-vertex / touched-cell lists of a general step; the same words
     // are the zero-kept counters c1 / c2 of a singleton step (a general step
-    // clears the entries it wrote)
-    uint32_t tvw[MAXN / 2 + 1], tclw[MAXN / 2 + 1];
+    // clears the entries it wrote).  With the lists in global scratch
+    // (kListG), c1 and c2 are 8-bit counters, two per 16-bit entry of c12
+    // (at most 64 each: one per neighbour of the splitter).
+    uint32_t tvw[NLW], tclw[NLW];
+    uint32_t c12[NCW];
     uint32_t rowl[NROW];
     uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
     uint64_t split[NW], oldend[NW];  // per general step: split cells (by first), their old lasts
     uint32_t ctr[4];
     const int64_t* rowp;
     int64_t rbase;  // rowp[0]
+    ix_t* tvg;      // the lists in global scratch (kListG)
+    ix_t* tclg;
 };
 
 // Hot state in LDS: addresses are compile-time constants.
@@ -464,12 +484,47 @@ struct HotL {
     __device__ __forceinline__ ix_t& ce(uint32_t i) { return S().ce[i]; }
     __device__ __forceinline__ ix_t& tmp(uint32_t i) { return S().tmp[i]; }
     __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().tmp[i]; }
-    __device__ __forceinline__ ix_t& tv(uint32_t i) { return ((ix_t*)S().tvw)[i]; }
-    __device__ __forceinline__ ix_t& tcl(uint32_t i) { return ((ix_t*)S().tclw)[i]; }
+    static constexpr bool kListG = LdsHot<MAXN>::kListG;
+    // the lists' words double as the singleton step's counters: a general
+    // step zeroes the list entries it wrote
+    static constexpr bool kZeroLists = !kListG;
+    __device__ __forceinline__ ix_t& tv(uint32_t i) {
+        if constexpr (kListG) return S().tvg[i];
+        else return ((ix_t*)S().tvw)[i];
+    }
+    __device__ __forceinline__ ix_t& tcl(uint32_t i) {
+        if constexpr (kListG) return S().tclg[i];
+        else return ((ix_t*)S().tclw)[i];
+    }
     GCL_PACKED16(cnt, S().cnt)
     GCL_PACKED16(hits, S().hits)
-    GCL_PACKED16(c1, S().tvw)
-    GCL_PACKED16(c2, S().tclw)
+    __device__ __forceinline__ uint32_t c1_add(uint32_t i) {
+        if constexpr (kListG) {
+            const uint32_t sh = (i & 1u) * 16u;
+            return (atomicAdd(&S().c12[i >> 1], 1u << sh) >> sh) & 0xFFu;
+        } else {
+            const uint32_t sh = (i & 1u) * 16u;
+            return (atomicAdd(&S().tvw[i >> 1], 1u << sh) >> sh) & 0xFFFFu;
+        }
+    }
+    __device__ __forceinline__ uint32_t c2_add(uint32_t i) {
+        if constexpr (kListG) {
+            const uint32_t sh = (i & 1u) * 16u + 8u;
+            return (atomicAdd(&S().c12[i >> 1], 1u << sh) >> sh) & 0xFFu;
+        } else {
+            const uint32_t sh = (i & 1u) * 16u;
+            return (atomicAdd(&S().tclw[i >> 1], 1u << sh) >> sh) & 0xFFFFu;
+        }
+    }
+    // both singleton counters of cell i back to zero
+    __device__ __forceinline__ void c12_clear(uint32_t i) {
+        if constexpr (kListG) {
+            ((uint16_t*)S().c12)[i] = 0;
+        } else {
+            ((uint16_t*)S().tvw)[i] = 0;
+            ((uint16_t*)S().tclw)[i] = 0;
+        }
+    }
     __device__ __forceinline__ uint64_t* split() { return S().split; }
     __device__ __forceinline__ uint64_t* oldend() { return S().oldend; }
     __device__ __forceinline__ uint64_t* active() { return S().active; }
@@ -483,6 +538,18 @@ struct HotL {
     }
     __device__ __forceinline__ const int64_t* rows() { return S().rowp; }
     __device__ __forceinline__ void sync() { wave_sync(); }
+    // order list writes before other lanes' reads (global lists: the
+    // stores must have completed)
+    __device__ __forceinline__ void lsync() {
+        if constexpr (kListG) group_sync();
+        else wave_sync();
+    }
+    __device__ __forceinline__ void bind_lists(uint8_t* p, uint32_t maxn) {
+        if constexpr (kListG) {
+            S().tvg = (ix_t*)p;
+            S().tclg = (ix_t*)p + maxn + 2;
+        }
+    }
     __device__ __forceinline__ void bind(const int64_t* rowp, uint32_t n, uint8_t*, uint32_t) {
         if (kRowLds) {
             for (uint32_t i = (uint32_t)lane_id(); i <= n; i += kWave) S().rowl[i] = (uint32_t)(rowp[i] - rowp[0]);
@@ -523,6 +590,13 @@ struct HotG {
     GCL_PACKED16(hits, h32 + (n1 / 2 + 1))
     GCL_PACKED16(c1, (uint32_t*)(h16 + 4 * n1))
     GCL_PACKED16(c2, (uint32_t*)(h16 + 5 * n1))
+    static constexpr bool kZeroLists = true;
+    __device__ __forceinline__ void c12_clear(uint32_t i) {
+        c1_st(i, 0);
+        c2_st(i, 0);
+    }
+    __device__ __forceinline__ void lsync() { group_sync(); }
+    __device__ __forceinline__ void bind_lists(uint8_t*, uint32_t) {}
     __device__ __forceinline__ uint64_t* split() { return h64; }
     __device__ __forceinline__ uint64_t* oldend() { return h64 + nw1; }
     __device__ __forceinline__ uint64_t* active() { return B().active; }
@@ -786,8 +860,7 @@ struct Worker {
                 }
                 H.cnt_st(p, 0);
                 H.hits_st(p, 0);
-                H.c1_st(p, 0);
-                H.c2_st(p, 0);
+                H.c12_clear(p);
             }
             const uint64_t b = ballot(is_last);
             if (l == 0) H.lasts()[base >> 6] = b;
@@ -858,8 +931,7 @@ struct Worker {
         uint32_t hkey = kNoKey;
         if (owner) {
             H.hits_st(c, 0);
-            H.c1_st(c, 0);
-            H.c2_st(c, 0);
+            H.c12_clear(c);
         }
         if (own) {
             H.ce(c) = (ix_t)(f + k);  // the back fragment [f+k, c]
@@ -987,7 +1059,9 @@ struct Worker {
     // whole-cell counting sort leaves them (the order inside a fragment is
     // unobservable).  Returns false, having changed nothing but scratch in
     // tmp[f, c], when the cell does not qualify.
-    __device__ bool sparse_split(Ctx& X, uint32_t f, uint32_t c, uint32_t L, uint32_t ntv) {
+    // This front end finds the cell's touched members in the touched-vertex
+    // list of the step (ntv entries).
+    __device__ bool sparse_split_list(Ctx& X, uint32_t f, uint32_t c, uint32_t L, uint32_t ntv) {
         const int l = lane_id();
         // touched members of this cell, listed in tmp[f, f + tc)
         uint32_t tc = 0;
@@ -1007,14 +1081,31 @@ struct Worker {
         tc = uni(tc);
         if (tc == 0 || tc > (uint32_t)kWave || 4 * tc >= L) return false;
         H.sync();
+        const bool act = (uint32_t)l < tc;
+        const uint32_t u = act ? (uint32_t)H.tmp(f + (uint32_t)l) : 0u;
+        sparse_split_core(X, f, c, L, tc, act, u);
+        return true;
+    }
+    // The same when every touched vertex of the step is held by a lane: the
+    // lanes with `in` hold the cell's touched members u.
+    __device__ bool sparse_split_lanes(Ctx& X, uint32_t f, uint32_t c, uint32_t L, bool in, uint32_t u) {
+        const uint32_t tc = (uint32_t)__popcll(ballot(in));
+        if (tc == 0 || 4 * tc >= L) return false;
+        sparse_split_core(X, f, c, L, tc, in, u);
+        return true;
+    }
+    // tc touched members, one in each lane with `member` (u: the member)
+    __device__ void sparse_split_core(Ctx& X, uint32_t f, uint32_t c, uint32_t L, uint32_t tc, bool member, uint32_t u) {
+        const int l = lane_id();
         const uint32_t B = c - tc + 1;  // first position of the touched block
         const bool act = (uint32_t)l < tc;
-        uint32_t u = 0, q = 0, hv = 0;
+        uint32_t q = 0, hv = 0;
         bool out = false, hole = false;
-        if (act) {
-            u = H.tmp(f + (uint32_t)l);
+        if (member) {
             q = H.pos(u);
             out = q < B;
+        }
+        if (act) {
             hv = H.elem(B + (uint32_t)l);
             hole = H.cnt_ld(hv) == 0u;
         }
@@ -1072,7 +1163,6 @@ struct Worker {
         }
         X.ncells += ngroups;
         H.sync();
-        return true;
     }
 
     // Hash chain and active-set rules of a step with a splitter of more than
@@ -1182,196 +1272,89 @@ struct Worker {
         for (uint32_t b = (uint32_t)l; b < nval; b += kWave) H.hits_st(b, 0);
     }
 
-    // General refinement step: splitter cell [p, e] of any size (reference
-    // src/partition_refinement.cpp:197-473): counts with LDS atomics; touched
-    // cells regrouped by key (count for |W| > 1; neighbours first for
-    // |W| == 1), small cells one per lane, large ones by the whole wave; then
-    // the hash chain and active-set rules over the pre-step non-singleton
-    // cells in position order.
-    __device__ void general_step(Ctx& X, uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
+    // Counting-sort split of a large touched cell [f, c] by its members'
+    // keys (count for |W| > 1; neighbours first for |W| == 1); a cell whose
+    // keys are all equal stays as it is.  hits is all zero on entry and on
+    // exit: it holds the buckets (indices 0..R-1).
+    __device__ void split_big_cell(Ctx& X, uint32_t f, uint32_t c, uint32_t s) {
         const int l = lane_id();
-        const uint32_t s = e - p + 1;
-        // counts: cnt[u] = neighbours of u in the splitter; the first touch of
-        // a vertex appends it to the tv list (ranks from ballots)
-        uint32_t ntv = 0;
-        uint64_t degsum = 0;
-        for (uint32_t base = 0; base < s; base += kWave) {
+        const uint32_t L = c - f + 1;
+        uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
+        for (uint32_t base = 0; base < L; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            uint32_t a = 0, d = 0;
-            if (i < s) {
-                const uint32_t w = H.elem(p + i);
-                a = xrow(X, w);
-                d = xrow(X, w + 1) - a;
-            }
-            const uint32_t incl = wave_scan(d);
-            const uint32_t total = readlane(incl, kWave - 1);
-            degsum += total;
-            const uint32_t nv = min(s - base, (uint32_t)kWave);
-            if (total <= (uint32_t)kWave) {
-                // one (vertex, neighbour) pair per lane: lane t takes the
-                // first vertex whose degree run [incl - d, incl) ends after t
-                const uint32_t off = a + d - incl;  // col index of a pair = t + off of its vertex
-                uint32_t x = 0;
-                bool found = false;
-                for (uint32_t j = 0; j < nv; ++j) {
-                    const uint32_t ij = readlane(incl, (int)j);
-                    const uint32_t oj = readlane(off, (int)j);
-                    if (!found && (uint32_t)l < ij) {
-                        found = true;
-                        x = (uint32_t)l + oj;
-                    }
-                }
-                bool first = false;
-                uint32_t u = 0;
-                if ((uint32_t)l < total) {
-                    u = (uint32_t)gload(X.col + x);
-                    first = H.cnt_add(u) == 0u;
-                }
-                const uint64_t fm = ballot(first);
-                if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
-                ntv += (uint32_t)__popcll(fm);
-            } else {
-                const uint32_t dmax = wave_max(d);
-                for (uint32_t t = 0; t < dmax; ++t) {
-                    bool first = false;
-                    uint32_t u = 0;
-                    if (t < d) {
-                        u = (uint32_t)gload(X.col + a + t);
-                        first = H.cnt_add(u) == 0u;
-                    }
-                    const uint64_t fm = ballot(first);
-                    if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
-                    ntv += (uint32_t)__popcll(fm);
-                }
+            if (i < L) {
+                const uint32_t v = H.elem(f + i);
+                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                kmin = min(kmin, kv);
+                kmax = max(kmax, kv);
             }
         }
-        X.csr_bytes += 12ull * s + 4ull * degsum;
-        H.sync();
-        // touched non-singleton cells, each once (hits[c] counts its touches)
-        uint32_t ntc = 0;
-        for (uint32_t base = 0; base < ntv; base += kWave) {
+        kmin = uni(wave_min(kmin));
+        kmax = uni(wave_max(kmax));
+        if (kmin == kmax) return;  // uniform: nothing moves
+        X.part_bytes += 8ull * L;
+        const uint32_t R = kmax - kmin + 1;
+        for (uint32_t base = 0; base < L; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            bool firstc = false;
-            uint32_t c = 0;
-            if (i < ntv) {
-                const uint32_t u = H.tv(i);
-                c = cend(H.pos(u));
-                if (cfirst(c) != c) firstc = H.hits_add(c) == 0u;
+            if (i < L) {
+                const uint32_t v = H.elem(f + i);
+                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                H.hits_add(kv - kmin);
             }
-            const uint64_t cm = ballot(firstc);
-            if (firstc) H.tcl(ntc + lanes_below(cm)) = (ix_t)c;
-            ntc += (uint32_t)__popcll(cm);
         }
         H.sync();
-        // small cells: one lane each; large ones are compacted to the front
-        // of the touched list (entry nbig + rank <= i was already read)
-        uint32_t nbig = 0, nfrag_extra = 0;
-        for (uint32_t base = 0; base < ntc; base += kWave) {
-            const uint32_t i = base + (uint32_t)l;
-            bool big = false, small = false;
-            uint32_t c = 0, f = 0;
-            if (i < ntc) {
-                c = H.tcl(i);
-                f = cfirst(c);
-                H.hits_st(c, 0);
-                small = c - f + 1 <= kSmallCell;
-                big = !small;
-            }
-            // the 4-wide network when no small cell here has more than 4 members
-            uint32_t nf = 1;
-            if (ballot(small && c - f + 1 > 4)) {
-                if (small) nf = regroup_small<8>(f, c, s);
-            } else {
-                if (small) nf = regroup_small<4>(f, c, s);
-            }
-            nfrag_extra += nf - 1;
-            if (nf > 1) X.lane_split += c - f + 1;
-            const uint64_t bm = ballot(big);
-            if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
-            nbig += (uint32_t)__popcll(bm);
+        // bucket sizes -> exclusive offsets, in place
+        uint32_t run = 0, nfrag = 0;
+        for (uint32_t base = 0; base < R; base += kWave) {
+            const uint32_t b = base + (uint32_t)l;
+            const uint32_t sz = b < R ? H.hits_ld(b) : 0u;
+            const uint32_t inc = wave_scan(sz);
+            if (b < R) H.hits_st(b, run + inc - sz);
+            run += readlane(inc, kWave - 1);
+            nfrag += (uint32_t)__popcll(ballot(sz != 0));
         }
-        X.ncells += uni(wave_sum(nfrag_extra));
         H.sync();
-        // hits is all zero from here on: it holds the buckets of the
-        // counting sort below (indices 0..R-1, cleared after each cell)
-        for (uint32_t j = 0; j < nbig; ++j) {
-            const uint32_t c = uni((uint32_t)H.tcl(j));
-            const uint32_t f = uni(cfirst(c));
-            const uint32_t L = c - f + 1;
-            if (s > 1 && sparse_split(X, f, c, L, ntv)) {
-                X.part_bytes += 8ull * L;
-                continue;
+        // each member takes the next slot of its bucket (the order inside
+        // a fragment is unobservable); offsets end as bucket ends
+        for (uint32_t base = 0; base < L; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < L) {
+                const uint32_t v = H.elem(f + i);
+                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                H.tmp(f + H.hits_add(kv - kmin)) = (ix_t)v;
             }
-            uint32_t kmin = 0xFFFFFFFFu, kmax = 0;
-            for (uint32_t base = 0; base < L; base += kWave) {
-                const uint32_t i = base + (uint32_t)l;
-                if (i < L) {
-                    const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    kmin = min(kmin, kv);
-                    kmax = max(kmax, kv);
-                }
-            }
-            kmin = uni(wave_min(kmin));
-            kmax = uni(wave_max(kmax));
-            if (kmin == kmax) continue;  // uniform: nothing moves
-            X.part_bytes += 8ull * L;
-            const uint32_t R = kmax - kmin + 1;
-            for (uint32_t base = 0; base < L; base += kWave) {
-                const uint32_t i = base + (uint32_t)l;
-                if (i < L) {
-                    const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    H.hits_add(kv - kmin);
-                }
-            }
-            H.sync();
-            // bucket sizes -> exclusive offsets, in place
-            uint32_t run = 0, nfrag = 0;
-            for (uint32_t base = 0; base < R; base += kWave) {
-                const uint32_t b = base + (uint32_t)l;
-                const uint32_t sz = b < R ? H.hits_ld(b) : 0u;
-                const uint32_t inc = wave_scan(sz);
-                if (b < R) H.hits_st(b, run + inc - sz);
-                run += readlane(inc, kWave - 1);
-                nfrag += (uint32_t)__popcll(ballot(sz != 0));
-            }
-            H.sync();
-            // each member takes the next slot of its bucket (the order inside
-            // a fragment is unobservable); offsets end as bucket ends
-            for (uint32_t base = 0; base < L; base += kWave) {
-                const uint32_t i = base + (uint32_t)l;
-                if (i < L) {
-                    const uint32_t v = H.elem(f + i);
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    H.tmp(f + H.hits_add(kv - kmin)) = (ix_t)v;
-                }
-            }
-            H.sync();
-            for (uint32_t base = 0; base < L; base += kWave) {
-                const uint32_t i = base + (uint32_t)l;
-                if (i < L) {
-                    const uint32_t P = f + i;
-                    const uint32_t v = H.tmp(P);
-                    H.elem(P) = (ix_t)v;
-                    H.pos(v) = (ix_t)P;
-                    const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                    const uint32_t b = kv - kmin;
-                    const uint32_t first = f + (b == 0 ? 0u : H.hits_ld(b - 1));
-                    const uint32_t last = f + H.hits_ld(b) - 1;
-                    H.ce(P) = (ix_t)ce_val(P, first, last);
-                    if (P == first && last != c) set_bit_atomic(H.lasts(), last);
-                }
-            }
-            X.ncells += nfrag - 1;
-            H.sync();
-            for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.hits_st(b, 0);
-            if (l == 0) {
-                set_bit_atomic(H.split(), f);
-                set_bit_atomic(H.oldend(), c);
-            }
-            H.sync();
         }
+        H.sync();
+        for (uint32_t base = 0; base < L; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            if (i < L) {
+                const uint32_t P = f + i;
+                const uint32_t v = H.tmp(P);
+                H.elem(P) = (ix_t)v;
+                H.pos(v) = (ix_t)P;
+                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
+                const uint32_t b = kv - kmin;
+                const uint32_t first = f + (b == 0 ? 0u : H.hits_ld(b - 1));
+                const uint32_t last = f + H.hits_ld(b) - 1;
+                H.ce(P) = (ix_t)ce_val(P, first, last);
+                if (P == first && last != c) set_bit_atomic(H.lasts(), last);
+            }
+        }
+        X.ncells += nfrag - 1;
+        H.sync();
+        for (uint32_t b = (uint32_t)l; b < R; b += kWave) H.hits_st(b, 0);
+        if (l == 0) {
+            set_bit_atomic(H.split(), f);
+            set_bit_atomic(H.oldend(), c);
+        }
+        H.sync();
+    }
+
+    // The end of a general step, once every touched cell is regrouped: the
+    // hash chain and active-set rules over the pre-step non-singleton cells
+    // in position order, and the cell masks of the new fragments.
+    __device__ void general_tail(Ctx& X, uint32_t s, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
         // compact list of the pre-step non-singleton cells (position order)
         uint32_t nns = 0;
         for (uint32_t base = 0; base < X.nw; base += kWave) {
@@ -1444,17 +1427,234 @@ struct Worker {
             }
         }
         H.sync();
-        // counts back to zero; the touched lists back to zero as well, since
+    }
+
+    // General refinement step: splitter cell [p, e] of any size (reference
+    // src/partition_refinement.cpp:197-473): counts with LDS atomics; touched
+    // cells regrouped by key (count for |W| > 1; neighbours first for
+    // |W| == 1), small cells one per lane, large ones by the whole wave; then
+    // the hash chain and active-set rules over the pre-step non-singleton
+    // cells in position order.
+    __device__ void general_step(Ctx& X, uint32_t p, uint32_t e, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        const uint32_t s = e - p + 1;
+        if (s <= (uint32_t)kWave) {
+            uint32_t a = 0, d = 0;
+            if ((uint32_t)l < s) {
+                const uint32_t w = H.elem(p + (uint32_t)l);
+                a = xrow(X, w);
+                d = xrow(X, w + 1) - a;
+            }
+            const uint32_t incl = wave_scan(d);
+            const uint32_t total = readlane(incl, kWave - 1);
+            if (total <= (uint32_t)kWave) {
+                general_step_lanes(X, s, a, d, incl, total, longcode, hint);
+                return;
+            }
+        }
+        general_step_list(X, p, s, longcode, hint);
+    }
+
+    // General step with at most a wave of splitter members and of (member,
+    // neighbour) pairs (a, d, incl: each member lane's row offset, degree and
+    // inclusive degree sum; total: the pairs).  A touched vertex stays in
+    // the lane that counted it first, and the first of those lanes in each
+    // touched non-singleton cell owns the cell: it regroups a small cell
+    // itself and names a large one to the wave.  No touched lists are kept.
+    __device__ void general_step_lanes(Ctx& X, uint32_t s, uint32_t a, uint32_t d, uint32_t incl, uint32_t total,
+                                       uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        // one (member, neighbour) pair per lane: lane t takes the first member
+        // whose degree run [incl - d, incl) ends after t
+        const uint32_t off = a + d - incl;  // col index of a pair = t + off of its member
+        uint32_t x = 0;
+        bool found = false;
+        for (uint32_t j = 0; j < s; ++j) {
+            const uint32_t ij = readlane(incl, (int)j);
+            const uint32_t oj = readlane(off, (int)j);
+            if (!found && (uint32_t)l < ij) {
+                found = true;
+                x = (uint32_t)l + oj;
+            }
+        }
+        bool first = false;
+        uint32_t u = 0;
+        if ((uint32_t)l < total) {
+            u = (uint32_t)gload(X.col + x);
+            first = H.cnt_add(u) == 0u;
+        }
+        X.csr_bytes += 12ull * s + 4ull * total;
+        H.sync();
+        // the counts are final: the cell of each touched vertex, and its owner
+        uint32_t c = 0, f = 0;
+        bool own = false;
+        if (first) {
+            c = cend(H.pos(u));
+            f = cfirst(c);
+            if (f != c) own = H.hits_add(c) == 0u;
+        }
+        H.sync();
+        bool small = false, big = false;
+        if (own) {
+            H.hits_st(c, 0);
+            small = c - f + 1 <= kSmallCell;
+            big = !small;
+        }
+        // the 4-wide network when no small cell has more than 4 members
+        uint32_t nf = 1;
+        if (ballot(small && c - f + 1 > 4)) {
+            if (small) nf = regroup_small<8>(f, c, s);
+        } else {
+            if (small) nf = regroup_small<4>(f, c, s);
+        }
+        if (nf > 1) X.lane_split += c - f + 1;
+        X.ncells += uni(wave_sum(nf - 1));
+        H.sync();
+        // hits is all zero from here on (the buckets of split_big_cell)
+        for (uint64_t bm = ballot(big); bm; bm &= bm - 1) {
+            const int j = __builtin_ctzll(bm);
+            const uint32_t cb = uni(readlane(c, j)), fb = uni(readlane(f, j));
+            const uint32_t L = cb - fb + 1;
+            if (s > 1 && sparse_split_lanes(X, fb, cb, L, first && c == cb, u)) {
+                X.part_bytes += 8ull * L;
+                continue;
+            }
+            split_big_cell(X, fb, cb, s);
+        }
+        general_tail(X, s, longcode, hint);
+        if (first) H.cnt_st(u, 0);  // counts back to zero
+        H.sync();
+    }
+
+    // General step for any splitter: the touched vertices and touched cells
+    // are listed in tv and tcl.
+    __device__ void general_step_list(Ctx& X, uint32_t p, uint32_t s, uint32_t& longcode, int& hint) {
+        const int l = lane_id();
+        // counts: cnt[u] = neighbours of u in the splitter; the first touch of
+        // a vertex appends it to the tv list (ranks from ballots)
+        uint32_t ntv = 0;
+        uint64_t degsum = 0;
+        for (uint32_t base = 0; base < s; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            uint32_t a = 0, d = 0;
+            if (i < s) {
+                const uint32_t w = H.elem(p + i);
+                a = xrow(X, w);
+                d = xrow(X, w + 1) - a;
+            }
+            const uint32_t incl = wave_scan(d);
+            const uint32_t total = readlane(incl, kWave - 1);
+            degsum += total;
+            const uint32_t nv = min(s - base, (uint32_t)kWave);
+            if (total <= (uint32_t)kWave) {
+                // one (vertex, neighbour) pair per lane: lane t takes the
+                // first vertex whose degree run [incl - d, incl) ends after t
+                const uint32_t off = a + d - incl;  // col index of a pair = t + off of its vertex
+                uint32_t x = 0;
+                bool found = false;
+                for (uint32_t j = 0; j < nv; ++j) {
+                    const uint32_t ij = readlane(incl, (int)j);
+                    const uint32_t oj = readlane(off, (int)j);
+                    if (!found && (uint32_t)l < ij) {
+                        found = true;
+                        x = (uint32_t)l + oj;
+                    }
+                }
+                bool first = false;
+                uint32_t u = 0;
+                if ((uint32_t)l < total) {
+                    u = (uint32_t)gload(X.col + x);
+                    first = H.cnt_add(u) == 0u;
+                }
+                const uint64_t fm = ballot(first);
+                if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
+                ntv += (uint32_t)__popcll(fm);
+            } else {
+                const uint32_t dmax = wave_max(d);
+                for (uint32_t t = 0; t < dmax; ++t) {
+                    bool first = false;
+                    uint32_t u = 0;
+                    if (t < d) {
+                        u = (uint32_t)gload(X.col + a + t);
+                        first = H.cnt_add(u) == 0u;
+                    }
+                    const uint64_t fm = ballot(first);
+                    if (first) H.tv(ntv + lanes_below(fm)) = (ix_t)u;
+                    ntv += (uint32_t)__popcll(fm);
+                }
+            }
+        }
+        X.csr_bytes += 12ull * s + 4ull * degsum;
+        H.lsync();
+        // touched non-singleton cells, each once (hits[c] counts its touches)
+        uint32_t ntc = 0;
+        for (uint32_t base = 0; base < ntv; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            bool firstc = false;
+            uint32_t c = 0;
+            if (i < ntv) {
+                const uint32_t u = H.tv(i);
+                c = cend(H.pos(u));
+                if (cfirst(c) != c) firstc = H.hits_add(c) == 0u;
+            }
+            const uint64_t cm = ballot(firstc);
+            if (firstc) H.tcl(ntc + lanes_below(cm)) = (ix_t)c;
+            ntc += (uint32_t)__popcll(cm);
+        }
+        H.lsync();
+        // small cells: one lane each; large ones are compacted to the front
+        // of the touched list (entry nbig + rank <= i was already read)
+        uint32_t nbig = 0, nfrag_extra = 0;
+        for (uint32_t base = 0; base < ntc; base += kWave) {
+            const uint32_t i = base + (uint32_t)l;
+            bool big = false, small = false;
+            uint32_t c = 0, f = 0;
+            if (i < ntc) {
+                c = H.tcl(i);
+                f = cfirst(c);
+                H.hits_st(c, 0);
+                small = c - f + 1 <= kSmallCell;
+                big = !small;
+            }
+            // the 4-wide network when no small cell here has more than 4 members
+            uint32_t nf = 1;
+            if (ballot(small && c - f + 1 > 4)) {
+                if (small) nf = regroup_small<8>(f, c, s);
+            } else {
+                if (small) nf = regroup_small<4>(f, c, s);
+            }
+            nfrag_extra += nf - 1;
+            if (nf > 1) X.lane_split += c - f + 1;
+            const uint64_t bm = ballot(big);
+            if (big) H.tcl(nbig + lanes_below(bm)) = (ix_t)c;
+            nbig += (uint32_t)__popcll(bm);
+        }
+        X.ncells += uni(wave_sum(nfrag_extra));
+        H.lsync();
+        // hits is all zero from here on (the buckets of split_big_cell)
+        for (uint32_t j = 0; j < nbig; ++j) {
+            const uint32_t c = uni((uint32_t)H.tcl(j));
+            const uint32_t f = uni(cfirst(c));
+            const uint32_t L = c - f + 1;
+            if (s > 1 && sparse_split_list(X, f, c, L, ntv)) {
+                X.part_bytes += 8ull * L;
+                continue;
+            }
+            split_big_cell(X, f, c, s);
+        }
+        general_tail(X, s, longcode, hint);
+        // counts back to zero; the touched lists back to zero as well where
         // their words are the singleton step's counters
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
             if (i < ntv) {
                 H.cnt_st(H.tv(i), 0);
-                H.tv(i) = 0;
+                if constexpr (Hot::kZeroLists) H.tv(i) = 0;
             }
         }
-        for (uint32_t i = (uint32_t)l; i < ntc; i += kWave) H.tcl(i) = 0;
-        H.sync();
+        if constexpr (Hot::kZeroLists)
+            for (uint32_t i = (uint32_t)l; i < ntc; i += kWave) H.tcl(i) = 0;
+        H.lsync();
     }
 
     // 8 bytes per vertex of the split cells counted in the lanes (every lane
@@ -2641,8 +2841,7 @@ struct Worker {
             H.elem(p) = re[p];
             H.cnt_st(p, 0);
             H.hits_st(p, 0);
-            H.c1_st(p, 0);
-            H.c2_st(p, 0);
+            H.c12_clear(p);
         }
         for (uint32_t w = (uint32_t)l; w < nw; w += kWave) {
             H.lasts()[w] = rs[w];
@@ -2886,8 +3085,7 @@ struct Worker {
             H.elem(p) = re[p];
             H.cnt_st(p, 0);
             H.hits_st(p, 0);
-            H.c1_st(p, 0);
-            H.c2_st(p, 0);
+            H.c12_clear(p);
         }
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) {
             H.lasts()[w] = rl[w];
@@ -3084,6 +3282,7 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.arena_cap = P.arena_cap;
     W.ring = P.ring;
     W.bind_auts(P.aut_pool ? nullptr : base + L.auts);
+    W.H.bind_lists(P.lists ? P.lists + (uint64_t)blockIdx.x * P.lists_stride : base + L.lists, P.maxn);
     W.levels = L.levels;
     W.step_budget = P.step_budget;
     const uint64_t now = wall_clock64();

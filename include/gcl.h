@@ -128,8 +128,9 @@ gcl_status gcl_canon_label(const gcl_csr* g, uint32_t max_aut, gcl_engine engine
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Every device buffer the search needs (launch parameters, counters, the
- * staged CSR, labels, result record, search scratch and one speculation
- * record) lives in a static device array of 6 MiB that each device running
+ * staged CSR, labels, result record, search scratch, one speculation
+ * record and the helpers' touched lists) lives in a static device array of
+ * 8 MiB that each device running
  * this call holds until the process ends; the call makes no hipMalloc,
  * hipFree or other allocating device call.  Limits: n <= 1024 and
  * nnz <= 32768 (GCL_E_CAPACITY beyond); automorphism ring of
