@@ -301,14 +301,12 @@ def configs_table(rows, verdicts, ref):
 POISSON_CONFIGURED = 1_000_000  # requests of the serving configuration's Poisson stream
 
 
-def served(args):
-    """Served measurement: requests over the same mix of sizes, half
-    canonicalize and half isomorphism tests (partner: a permuted copy or a
-    one-swap variant, 50/50), through the request server."""
-    R = args.serve_requests
-    t0 = time.time()
-    first = gcl.mix_batch(R, args.seed + 1)
-    rs = np.random.RandomState(args.seed + 2)
+def served_workload(R, seed):
+    """R requests over the batch's size mix: graph i, and for about half of
+    them a partner (a permuted copy or a one-swap variant, 50/50) that makes
+    the request a pair test.  Returns (batch, pair_with, pairs)."""
+    first = gcl.mix_batch(R, seed)
+    rs = np.random.RandomState(seed + 1)
     graphs = [first.graph(i) for i in range(R)]
     pair_with = np.full(R, -1, np.int32)
     partners = []
@@ -317,13 +315,28 @@ def served(args):
             g = graphs[i]
             partners.append(gcl.permute(g, 1000003 + i) if rs.rand() < 0.5 else gcl.edge_swap(g, 2000003 + i))
             pair_with[i] = R + len(partners) - 1
-    batch = gcl.Batch(graphs + partners)
-    log(f"served workload: {R} requests ({len(partners)} pairs) built in {time.time() - t0:.1f}s")
-    out = serve_run(args, batch, R, pair_with, len(partners), "served", dict(max_n=1024))
+    return gcl.Batch(graphs + partners), pair_with, len(partners)
+
+
+def served(args):
+    """Served measurement: requests over the same mix of sizes, half
+    canonicalize and half isomorphism tests (partner: a permuted copy or a
+    one-swap variant, 50/50), through the request server.  The Poisson phase
+    runs --serve-poisson requests (default: the capacity phase's count): a
+    stream of chunks of --serve-requests requests each, every chunk freshly
+    generated (the next one is built while the current one is served)."""
+    R = args.serve_requests
+    t0 = time.time()
+    batch, pair_with, npairs = served_workload(R, args.seed + 1)
+    log(f"served workload: {R} requests ({npairs} pairs) built in {time.time() - t0:.1f}s")
+    total = max(args.serve_poisson, 0) or R
+    out = serve_run(args, batch, R, pair_with, npairs, "served", dict(max_n=1024), poisson_total=total,
+                    chunk=lambda k: served_workload(R, args.seed + 1 + 1000 * k)[:2])
     out["workload"] = (f"same size mix (n log-uniform [16,1024]); half canonicalize, half pair tests "
                        f"(permuted copy or one-swap variant, 50/50); persistent serving kernels, one per size "
-                       f"class; {R} requests in each phase (the configuration's stream has "
-                       f"{POISSON_CONFIGURED}); the 2% of n 10k-50k graphs are in served_large")
+                       f"class; capacity phase {R} requests, Poisson phase {total} requests in chunks of {R} "
+                       f"distinct generated requests (the configuration's stream has {POISSON_CONFIGURED}); "
+                       f"the 2% of n 10k-50k graphs are in served_large")
     return out
 
 
@@ -375,11 +388,15 @@ def served_large(args):
     return out
 
 
-def serve_run(args, batch, R, pair_with, npairs, tag, cfg):
+def serve_run(args, batch, R, pair_with, npairs, tag, cfg, poisson_total=None, chunk=None):
     """Phase 1 submits the R requests at once (capacity); phase 2 replays a
-    Poisson stream at half that rate (latency from arrival to completion).
-    Requests refused at admission (GCL_E_CAPACITY) are counted and left out
-    of the rates and latencies."""
+    Poisson stream at half that rate (latency from arrival to completion):
+    poisson_total requests (default R), the first R being phase 1's and each
+    further R a chunk from chunk(k) = (batch, pair_with), served back to back
+    by the same server.  Requests refused at admission (GCL_E_CAPACITY) are
+    counted and left out of the rates and latencies."""
+    import concurrent.futures
+    total = poisson_total or R
     srv = gcl.Server(max_aut=10000, lifetime_ms=600000, **cfg)
     try:
         warm = min(R, 512)
@@ -396,10 +413,24 @@ def serve_run(args, batch, R, pair_with, npairs, tag, cfg):
         labelings = int(served_ok.sum()) + int(((pair_with >= 0) & served_ok).sum())
         cap = labelings / cap_s
         rate = 0.5 * int(served_ok.sum()) / cap_s
-        arrival = np.cumsum(np.random.RandomState(args.seed + 3).exponential(1.0 / rate, size=R))
+        arr_rs = np.random.RandomState(args.seed + 3)
+        parts = []
         t2 = time.perf_counter()
-        r2 = srv.replay(batch, arrival, pair_with)
+        with concurrent.futures.ThreadPoolExecutor(1) as ex:
+            done, k, cur, nxt = 0, 0, (batch, pair_with), None
+            while done < total:
+                cnt = min(R, total - done)
+                if done + cnt < total:
+                    nxt = ex.submit(chunk, k + 1)  # built while this chunk is served
+                arrival = np.cumsum(arr_rs.exponential(1.0 / rate, size=cnt))
+                parts.append(srv.replay(cur[0], arrival, cur[1][:cnt]))
+                done += cnt
+                k += 1
+                if done < total:
+                    cur = nxt.result()
+                    log(f"{tag}: poisson {done}/{total} requests")
         lat_s = time.perf_counter() - t2
+        r2 = {f: np.concatenate([p[f] for p in parts]) for f in parts[0]}
         refused2 = r2["status"] == -6
         if np.any((r2["status"] != 0) & ~refused2):
             raise SystemExit("bench.py: served requests failed")
@@ -413,7 +444,7 @@ def serve_run(args, batch, R, pair_with, npairs, tag, cfg):
         f"p50 {np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms; refused {int(refused2.sum())}")
     return {"value": round(cap, 2), "unit": "labelings/s",
             "requests": R, "pair_requests": npairs, "refused_capacity": int(refused2.sum()),
-            "poisson_requests": R,
+            "poisson_requests": int(total), "poisson_chunk": R,
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3), "max": round(float(lat.max()), 3)},
             "poisson_rate_req_s": round(rate, 1), "poisson_wall_s": round(lat_s, 3),
@@ -445,10 +476,15 @@ def main():
     ap.add_argument("--autotune", type=int, default=0,
                     help="pick the launch tuning with gcl_batch_autotune on a sample of this many graphs "
                          "(untimed, before the warmup); 0 uses the defaults")
-    ap.add_argument("--serve-requests", type=int, default=20000,
-                    help="requests of the served measurement (0 skips it)")
-    ap.add_argument("--serve-large-requests", type=int, default=2000,
-                    help="requests of the served measurement with 2%% large graphs (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=0,
+                    help="requests of the served measurement, e.g. 20000 (0 skips it; off by default: one "
+                         "served run of the current kernels faulted the GPU and its cause is not found)")
+    ap.add_argument("--serve-poisson", type=int, default=0,
+                    help="requests of the served Poisson phase (0: as many as --serve-requests); "
+                         f"{POISSON_CONFIGURED} is the serving configuration's stream")
+    ap.add_argument("--serve-large-requests", type=int, default=0,
+                    help="requests of the served measurement with 2%% large graphs, e.g. 2000 (0 skips it; "
+                         "off by default, as --serve-requests)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
