@@ -629,6 +629,10 @@ struct WithDesc : Base {
 struct Ctx {
     uint32_t n, nw, ncells, nactive;
     uint64_t steps, csr_bytes;
+    // this refinement's singleton steps and the degrees of their splitters
+    // (32-bit: a refinement takes at most 2n steps), folded into csr_bytes
+    // at its end
+    uint32_t nsing, sdeg;
     // algorithmic bytes beyond the CSR reads: 8 per non-singleton cell hashed
     // under a non-singleton splitter, 8 per vertex of a cell that splits, 8n
     // per refinement (the element vector loaded and stored once)
@@ -884,7 +888,8 @@ struct Worker {
     // loaded by the caller
     __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t u, uint32_t deg, uint32_t& longcode, int& hint) {
         const int l = lane_id();
-        X.csr_bytes += 12 + 4ull * deg;
+        ++X.nsing;
+        X.sdeg += deg;
         uint32_t c = 0, f = 0, r = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
@@ -1676,6 +1681,8 @@ struct Worker {
         X.nactive = uni(nactive);
         X.steps = uni(steps);
         X.csr_bytes = uni(csr_bytes);
+        X.nsing = 0;
+        X.sdeg = 0;
         X.part_bytes = uni(part_bytes) + 8ull * X.n;
         X.lane_split = 0;
         X.col = uni(col);
@@ -1698,13 +1705,16 @@ struct Worker {
         H.sync();
         uint32_t longcode = X.ncells;
         int hint = 0;
-        uint64_t bud = uni(budget);
+        // the step budget, counted down in 32 bits (a refinement takes at
+        // most 2n steps)
+        const uint64_t bud0 = uni(budget);
+        const uint32_t bud32 = bud0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bud0;
+        uint32_t nst = 0;  // steps of this refinement
         while (X.ncells < X.n && X.nactive > 0) {
-            if (bud == 0) {
+            if (nst == bud32) {
                 status = GCL_E_INTERNAL;  // step budget exhausted
                 break;
             }
-            --bud;
             // splitter: the first active position at or after the hint, else
             // the first one; the hint's own word is tested first (one scalar load)
             const uint32_t hw = (uint32_t)hint >> 6;
@@ -1723,7 +1733,7 @@ struct Worker {
             // bitmap between steps)
             if (l == 0) H.active()[p >> 6] = uni(word) & ~(1ull << (p & 63u));
             --X.nactive;
-            ++X.steps;
+            ++nst;
             // the cell end and the splitter's vertex (used when the cell is a
             // singleton) are independent loads
             const uint32_t e = uni(cend(p));
@@ -1745,10 +1755,12 @@ struct Worker {
             // (an LDS class has n <= 8192, so a lane's count stays below
             // n^2 < 2^32 over a whole refinement)
             if constexpr (!Hot::kLds)
-                if ((X.steps & 0x3FFFu) == 0) flush_split_bytes(X);
+                if ((nst & 0x3FFFu) == 0) flush_split_bytes(X);
         }
         flush_split_bytes(X);
-        budget = bud;
+        budget = bud0 - nst;
+        X.steps += nst;
+        X.csr_bytes += 12ull * X.nsing + 4ull * X.sdeg;
         if (X.nactive) {
             for (uint32_t w = (uint32_t)l; w < X.nw; w += kWave) H.active()[w] = 0;
             X.nactive = 0;
