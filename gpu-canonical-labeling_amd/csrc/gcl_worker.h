@@ -57,7 +57,7 @@ struct Layout {
     uint32_t levels;  // snapshot levels available
 };
 
-constexpr int kHot16 = 6;  // elem pos ce tmp(=sb) tv(=c1) tcl(=c2)
+constexpr int kHot16 = 6;  // elem pos ce tmp(=sb) tv tcl
 constexpr int kHot32 = 2;  // cnt hits
 constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
 // LDS classes from this size keep the touched-vertex and touched-cell lists
@@ -445,20 +445,17 @@ struct LdsHot {
     static constexpr int NROW = MAXN < 1024 ? MAXN + 1 : 1;
     static constexpr bool kListG = (uint32_t)MAXN >= kListsGlobalN;
     static constexpr int NLW = kListG ? 1 : MAXN / 2 + 1;
-    static constexpr int NCW = kListG ? MAXN / 2 + 1 : 1;
     // tmp is also sb: the staging area of a large-cell regroup and the list
-    // of non-singleton cells are used one after the other
+    // of non-singleton cells are used one after the other; in a singleton
+    // step it holds each touched cell's hole list and, at the cell's last
+    // position, its hole / outsider counters (Worker::singleton_step)
     // ce: the cell array (cell_end / cell_first merged, see Worker::cend)
     ix_t elem[MAXN], pos[MAXN], ce[MAXN], tmp[MAXN];
     // 16-bit counters packed in pairs (atomics on the containing word)
     uint32_t cnt[MAXN / 2 + 1], hits[MAXN / 2 + 1];
-    // touched-vertex / touched-cell lists of a general step; the same words
-    // are the zero-kept counters c1 / c2 of a singleton step (a general step
-    // clears the entries it wrote).  With the lists in global scratch
-    // (kListG), c1 and c2 are 8-bit counters, two per 16-bit entry of c12
-    // (at most 64 each: one per neighbour of the splitter).
+    // touched-vertex / touched-cell lists of a general step (in global
+    // scratch from kListsGlobalN vertices: kListG)
     uint32_t tvw[NLW], tclw[NLW];
-    uint32_t c12[NCW];
     uint32_t rowl[NROW];
     uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
     uint64_t split[NW], oldend[NW];  // per general step: split cells (by first), their old lasts
@@ -485,9 +482,6 @@ struct HotL {
     __device__ __forceinline__ ix_t& tmp(uint32_t i) { return S().tmp[i]; }
     __device__ __forceinline__ ix_t& sb(uint32_t i) { return S().tmp[i]; }
     static constexpr bool kListG = LdsHot<MAXN>::kListG;
-    // the lists' words double as the singleton step's counters: a general
-    // step zeroes the list entries it wrote
-    static constexpr bool kZeroLists = !kListG;
     __device__ __forceinline__ ix_t& tv(uint32_t i) {
         if constexpr (kListG) return S().tvg[i];
         else return ((ix_t*)S().tvw)[i];
@@ -498,32 +492,11 @@ struct HotL {
     }
     GCL_PACKED16(cnt, S().cnt)
     GCL_PACKED16(hits, S().hits)
-    __device__ __forceinline__ uint32_t c1_add(uint32_t i) {
-        if constexpr (kListG) {
-            const uint32_t sh = (i & 1u) * 16u;
-            return (atomicAdd(&S().c12[i >> 1], 1u << sh) >> sh) & 0xFFu;
-        } else {
-            const uint32_t sh = (i & 1u) * 16u;
-            return (atomicAdd(&S().tvw[i >> 1], 1u << sh) >> sh) & 0xFFFFu;
-        }
-    }
-    __device__ __forceinline__ uint32_t c2_add(uint32_t i) {
-        if constexpr (kListG) {
-            const uint32_t sh = (i & 1u) * 16u + 8u;
-            return (atomicAdd(&S().c12[i >> 1], 1u << sh) >> sh) & 0xFFu;
-        } else {
-            const uint32_t sh = (i & 1u) * 16u;
-            return (atomicAdd(&S().tclw[i >> 1], 1u << sh) >> sh) & 0xFFFFu;
-        }
-    }
-    // both singleton counters of cell i back to zero
-    __device__ __forceinline__ void c12_clear(uint32_t i) {
-        if constexpr (kListG) {
-            ((uint16_t*)S().c12)[i] = 0;
-        } else {
-            ((uint16_t*)S().tvw)[i] = 0;
-            ((uint16_t*)S().tclw)[i] = 0;
-        }
+    // adds inc to the 16-bit entry tmp[i] through its 32-bit word (the
+    // other half is untouched: the entry never carries); returns the old entry
+    __device__ __forceinline__ uint32_t tmp_add(uint32_t i, uint32_t inc) {
+        const uint32_t sh = (i & 1u) * 16u;
+        return (atomicAdd((uint32_t*)S().tmp + (i >> 1), inc << sh) >> sh) & 0xFFFFu;
     }
     __device__ __forceinline__ uint64_t* split() { return S().split; }
     __device__ __forceinline__ uint64_t* oldend() { return S().oldend; }
@@ -588,12 +561,10 @@ struct HotG {
     __device__ __forceinline__ ix_t& tcl(uint32_t i) { return h16[5 * n1 + i]; }
     GCL_PACKED16(cnt, h32)
     GCL_PACKED16(hits, h32 + (n1 / 2 + 1))
-    GCL_PACKED16(c1, (uint32_t*)(h16 + 4 * n1))
-    GCL_PACKED16(c2, (uint32_t*)(h16 + 5 * n1))
-    static constexpr bool kZeroLists = true;
-    __device__ __forceinline__ void c12_clear(uint32_t i) {
-        c1_st(i, 0);
-        c2_st(i, 0);
+    // tmp_add as in HotL (n1 is a multiple of 64, so tmp is word aligned)
+    __device__ __forceinline__ uint32_t tmp_add(uint32_t i, uint32_t inc) {
+        const uint32_t sh = (i & 1u) * 16u;
+        return (atomicAdd((uint32_t*)(h16 + 3 * n1) + (i >> 1), inc << sh) >> sh) & 0xFFFFu;
     }
     __device__ __forceinline__ void lsync() { group_sync(); }
     __device__ __forceinline__ void bind_lists(uint8_t*, uint32_t) {}
@@ -864,7 +835,6 @@ struct Worker {
                 }
                 H.cnt_st(p, 0);
                 H.hits_st(p, 0);
-                H.c12_clear(p);
             }
             const uint64_t b = ballot(is_last);
             if (l == 0) H.lasts()[base >> 6] = b;
@@ -890,36 +860,41 @@ struct Worker {
         const int l = lane_id();
         ++X.nsing;
         X.sdeg += deg;
-        uint32_t c = 0, f = 0, r = 0;
+        uint32_t c = 0, f = 0, r = 0, q = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
-            c = cend(H.pos(u));
-            f = cfirst(c);
-            ns = f != c;
+            q = H.pos(u);
+            // ce[q] == q exactly when q's cell is the singleton [q, q]
+            const uint32_t v = H.ce(q);
+            ns = v != q;
+            c = max(v, q);
         }
         if (ns) {
             H.cnt_st(u, 1);
             r = H.hits_add(c);
+            f = cfirst(c);
+            // the first toucher clears the cell's hole / outsider counters:
+            // tmp[c], outside the cell's hole list [f, f+k) once it splits
+            if (r == 0) H.tmp(c) = 0;
         }
         H.sync();
-        uint32_t k = 0, L = 0, q = 0;
-        bool split = false, out = false;
+        uint32_t k = 0, L = 0;
+        bool split = false, out = false, hole = false;
         if (ns) {
             k = H.hits_ld(c);
             L = c - f + 1;
             split = k < L;
-            q = H.pos(u);
             out = split && q >= f + k;
+            if (split) hole = H.cnt_ld(H.elem(f + r)) == 0;
         }
-        if (split) {
-            const uint32_t x = H.elem(f + r);
-            if (H.cnt_ld(x) == 0) {
-                const uint32_t h = H.c1_add(c);
-                H.tmp(f + h) = (ix_t)(f + r);
-            }
-        }
+        // one atomic per lane numbers its hole (low byte of tmp[c]) and its
+        // outsider (high byte); a cell has at most 64 of each
         uint32_t o = 0;
-        if (out) o = H.c2_add(c);
+        if (hole || out) {
+            const uint32_t old = H.tmp_add(c, (hole ? 1u : 0u) | (out ? 0x100u : 0u));
+            if (hole) H.tmp(f + (old & 0xFFu)) = (ix_t)(f + r);
+            o = old >> 8;
+        }
         H.sync();
         if (out) {  // move u into the front block over a non-neighbour
             const uint32_t hp = H.tmp(f + o);
@@ -934,10 +909,7 @@ struct Worker {
         const bool own = owner && split;
         if (split) H.ce(f + r) = (ix_t)ce_val(f + r, f, f + k - 1);  // the front fragment [f, f+k-1]
         uint32_t hkey = kNoKey;
-        if (owner) {
-            H.hits_st(c, 0);
-            H.c12_clear(c);
-        }
+        if (owner) H.hits_st(c, 0);
         if (own) {
             H.ce(c) = (ix_t)(f + k);  // the back fragment [f+k, c]
             // active-set rule (414-435): one new active cell per split
@@ -1648,17 +1620,11 @@ struct Worker {
             split_big_cell(X, f, c, s);
         }
         general_tail(X, s, longcode, hint);
-        // counts back to zero; the touched lists back to zero as well where
-        // their words are the singleton step's counters
+        // counts back to zero
         for (uint32_t base = 0; base < ntv; base += kWave) {
             const uint32_t i = base + (uint32_t)l;
-            if (i < ntv) {
-                H.cnt_st(H.tv(i), 0);
-                if constexpr (Hot::kZeroLists) H.tv(i) = 0;
-            }
+            if (i < ntv) H.cnt_st(H.tv(i), 0);
         }
-        if constexpr (Hot::kZeroLists)
-            for (uint32_t i = (uint32_t)l; i < ntc; i += kWave) H.tcl(i) = 0;
         H.lsync();
     }
 
@@ -2853,7 +2819,6 @@ struct Worker {
             H.elem(p) = re[p];
             H.cnt_st(p, 0);
             H.hits_st(p, 0);
-            H.c12_clear(p);
         }
         for (uint32_t w = (uint32_t)l; w < nw; w += kWave) {
             H.lasts()[w] = rs[w];
@@ -3097,7 +3062,6 @@ struct Worker {
             H.elem(p) = re[p];
             H.cnt_st(p, 0);
             H.hits_st(p, 0);
-            H.c12_clear(p);
         }
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) {
             H.lasts()[w] = rl[w];
