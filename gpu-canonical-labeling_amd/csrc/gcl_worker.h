@@ -63,8 +63,8 @@ constexpr int kHot64 = 2;  // split-cell bitmap, old-end bitmap
 // LDS classes from this size keep the touched-vertex and touched-cell lists
 // of a general step in global scratch (the step keeps them in its lanes when
 // it has at most a wave of (member, neighbour) pairs), which leaves LDS room
-// for one more worker per CU.
-constexpr uint32_t kListsGlobalN = 1024;
+// for more workers per CU.
+constexpr uint32_t kListsGlobalN = 512;
 // the touched-vertex and touched-cell lists of one worker (maxn + 2 entries each)
 __host__ __device__ inline uint64_t lists_bytes(uint32_t maxn) { return align256(2ull * 2ull * ((uint64_t)maxn + 2)); }
 // element stride of the global hot arrays: even, so that the 16-bit arrays
@@ -457,8 +457,12 @@ struct LdsHot {
     // scratch from kListsGlobalN vertices: kListG)
     uint32_t tvw[NLW], tclw[NLW];
     uint32_t rowl[NROW];
-    uint64_t active[NW], nsmask[NW], vmask[NW], lasts[NW];
-    uint64_t split[NW], oldend[NW];  // per general step: split cells (by first), their old lasts
+    // split / oldend: per general step, the split cells (by first position)
+    // and their old last positions, all zero between steps; split is also
+    // the vertex bitmap of a target cell's members (vmask), used between
+    // refinements only and cleared after each use
+    uint64_t active[NW], nsmask[NW], lasts[NW];
+    uint64_t split[NW], oldend[NW];
     uint32_t ctr[4];
     const int64_t* rowp;
     int64_t rbase;  // rowp[0]
@@ -502,7 +506,7 @@ struct HotL {
     __device__ __forceinline__ uint64_t* oldend() { return S().oldend; }
     __device__ __forceinline__ uint64_t* active() { return S().active; }
     __device__ __forceinline__ uint64_t* nsmask() { return S().nsmask; }
-    __device__ __forceinline__ uint64_t* vmask() { return S().vmask; }
+    __device__ __forceinline__ uint64_t* vmask() { return S().split; }
     __device__ __forceinline__ uint64_t* lasts() { return S().lasts; }
     __device__ __forceinline__ uint32_t* ctr() { return S().ctr; }
     __device__ __forceinline__ uint32_t row(uint32_t v) {
