@@ -696,6 +696,18 @@ struct Worker {
     __device__ __forceinline__ static bool test_bit(const uint64_t* m, uint32_t p) {
         return (m[p >> 6] >> (p & 63)) & 1ull;
     }
+    // 32-bit views of a bitmap (bit p in word p >> 5): one bit tested or
+    // updated without 64-bit shifts.  or_bits32 ors in b << (p & 31),
+    // and_bits32 keeps bit p only when b is 1 (b is 0 or 1).
+    __device__ __forceinline__ static bool test_bit32(const uint64_t* m, uint32_t p) {
+        return (((const uint32_t*)m)[p >> 5] >> (p & 31)) & 1u;
+    }
+    __device__ __forceinline__ static void or_bits32(uint64_t* m, uint32_t p, uint32_t b) {
+        atomicOr((uint32_t*)m + (p >> 5), b << (p & 31));
+    }
+    __device__ __forceinline__ static void and_bits32(uint64_t* m, uint32_t p, uint32_t b) {
+        atomicAnd((uint32_t*)m + (p >> 5), ~((1u - b) << (p & 31)));
+    }
     __device__ __forceinline__ static void set_bit_atomic(uint64_t* m, uint32_t p) {
         atomicOr((unsigned long long*)&m[p >> 6], (unsigned long long)(1ull << (p & 63)));
     }
@@ -867,19 +879,22 @@ struct Worker {
         uint32_t c = 0, f = 0, r = 0, q = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
+            // the neighbour mark (read only at positions of non-singleton
+            // cells, so a neighbour in a singleton cell may carry it too)
+            H.cnt_st(u, 1);
             q = H.pos(u);
             // ce[q] == q exactly when q's cell is the singleton [q, q]
             const uint32_t v = H.ce(q);
             ns = v != q;
             c = max(v, q);
-        }
-        if (ns) {
-            H.cnt_st(u, 1);
-            r = H.hits_add(c);
-            f = cfirst(c);
-            // the first toucher clears the cell's hole / outsider counters:
-            // tmp[c], outside the cell's hole list [f, f+k) once it splits
-            if (r == 0) H.tmp(c) = 0;
+            if (ns) {
+                r = H.hits_add(c);
+                f = cfirst(c);
+                // the first toucher clears the cell's hole / outsider
+                // counters: tmp[c], outside the cell's hole list [f, f+k)
+                // once it splits
+                if (r == 0) H.tmp(c) = 0;
+            }
         }
         H.sync();
         uint32_t k = 0, L = 0;
@@ -916,21 +931,23 @@ struct Worker {
         if (owner) H.hits_st(c, 0);
         if (own) {
             H.ce(c) = (ix_t)(f + k);  // the back fragment [f+k, c]
-            // active-set rule (414-435): one new active cell per split
-            uint32_t cand = 0xFFFFu;
-            if (test_bit(H.active(), f) || k >= L - k) {
-                set_bit_atomic(H.active(), f + k);
-                if (L - k == 1) cand = f + k;
-            } else {
-                set_bit_atomic(H.active(), f);
-                if (k == 1) cand = f;
-            }
-            if (cand != 0xFFFFu) hkey = ((f + 1) << 16) | cand;
-            if (k == 1) clear_bit_atomic(H.nsmask(), f);
-            if (L - k > 1) set_bit_atomic(H.nsmask(), f + k);
-            set_bit_atomic(H.lasts(), f + k - 1);
+            // active-set rule (414-435): one new active cell per split, the
+            // back fragment when the cell was active or the front one is not
+            // smaller, else the front one; a singleton one is the hint
+            // candidate
+            const bool back = test_bit32(H.active(), f) || k >= L - k;
+            const uint32_t a = back ? f + k : f;
+            or_bits32(H.active(), a, 1u);
+            if ((back ? L - k : k) == 1) hkey = ((f + 1) << 16) | a;
+            // non-singleton marks: f's stays unless the front fragment is a
+            // singleton; f+k's is set unless the back one is
+            and_bits32(H.nsmask(), f, k == 1 ? 0u : 1u);
+            or_bits32(H.nsmask(), f + k, L - k > 1 ? 1u : 0u);
+            or_bits32(H.lasts(), f + k - 1, 1u);
         }
-        if (ns) H.cnt_st(u, 0);
+        // counts back to zero (lanes without a neighbour hold u = 0, which
+        // is cleared as well: every count ends the step at zero)
+        H.cnt_st(u, 0);
         const uint64_t om = ballot(own);
         const uint32_t nsplit = (uint32_t)__popcll(om);
         if (nsplit == 0) {
