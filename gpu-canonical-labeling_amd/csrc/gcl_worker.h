@@ -698,7 +698,8 @@ struct Worker {
     }
     // 32-bit views of a bitmap (bit p in word p >> 5): one bit tested or
     // updated without 64-bit shifts.  or_bits32 ors in b << (p & 31),
-    // and_bits32 keeps bit p only when b is 1 (b is 0 or 1).
+    // and_bits32 keeps bit p only when b is 1, xor_bits32 toggles it when b
+    // is 1 (b is 0 or 1).
     __device__ __forceinline__ static bool test_bit32(const uint64_t* m, uint32_t p) {
         return (((const uint32_t*)m)[p >> 5] >> (p & 31)) & 1u;
     }
@@ -707,6 +708,9 @@ struct Worker {
     }
     __device__ __forceinline__ static void and_bits32(uint64_t* m, uint32_t p, uint32_t b) {
         atomicAnd((uint32_t*)m + (p >> 5), ~((1u - b) << (p & 31)));
+    }
+    __device__ __forceinline__ static void xor_bits32(uint64_t* m, uint32_t p, uint32_t b) {
+        atomicXor((uint32_t*)m + (p >> 5), b << (p & 31));
     }
     __device__ __forceinline__ static void set_bit_atomic(uint64_t* m, uint32_t p) {
         atomicOr((unsigned long long*)&m[p >> 6], (unsigned long long)(1ull << (p & 63)));
@@ -1187,14 +1191,15 @@ struct Worker {
             bool sp = false;
             if (valid) {
                 f = H.sb(i);
-                sp = test_bit(H.split(), f);
+                sp = test_bit32(H.split(), f);
                 nv = 1;
                 if (sp) {
+                    // its fragments: up to the old last position (oldend)
                     nv = 0;
                     for (uint32_t q = f;;) {
                         const uint32_t last = cend(q);
                         ++nv;
-                        if (test_bit(H.oldend(), last) || last + 1 >= X.n) break;
+                        if (test_bit32(H.oldend(), last) || last + 1 >= X.n) break;
                         q = last + 1;
                     }
                 }
@@ -1204,36 +1209,35 @@ struct Worker {
             int32_t cand = -1;
             if (valid && !sp) H.hits_st(k, H.cnt_ld(H.elem(f)) + f);
             if (sp) {
-                const bool was_active = test_bit(H.active(), f);
-                uint32_t max_size = 0, max_first = f;
-                for (uint32_t q = f;;) {
-                    const uint32_t last = cend(q);
+                // Every fragment after the first becomes active: its first
+                // position was inside the cell, so never active before, and
+                // each adds one active cell.  A cell that was not active
+                // becomes active at f and loses its largest fragment, which
+                // leaves the count as it is (reference 414-460).
+                const bool was_active = test_bit32(H.active(), f);
+                uint32_t max_size = 0, max_first = f, q = f, last = f;
+                for (uint32_t t = 0; t < nv; ++t) {
+                    last = cend(q);
                     const uint32_t size = last - q + 1;
                     H.hits_st(k++, H.cnt_ld(H.elem(q)) + q);
                     if (size > max_size) {
                         max_size = size;
                         max_first = q;
                     }
-                    if (q != f) {
-                        const unsigned long long bit = 1ull << (q & 63);
-                        if (!(atomicOr((unsigned long long*)&H.active()[q >> 6], bit) & bit)) ++dact;
-                        if (last == q) cand = (int32_t)q;
-                    }
-                    // the fragment's non-singleton mark
-                    if (last > q) set_bit_atomic(H.nsmask(), q);
-                    else clear_bit_atomic(H.nsmask(), q);
-                    if (test_bit(H.oldend(), last)) {
-                        clear_bit_atomic(H.oldend(), last);
-                        break;
-                    }
-                    if (last + 1 >= X.n) break;
+                    const bool later = q != f;
+                    or_bits32(H.active(), q, later ? 1u : 0u);
+                    if (later && last == q) cand = (int32_t)q;
+                    // the fragment's non-singleton mark (set before at f
+                    // only): toggled where it changes
+                    xor_bits32(H.nsmask(), q, (last > q) != !later ? 1u : 0u);
                     q = last + 1;
                 }
-                clear_bit_atomic(H.split(), f);
+                and_bits32(H.oldend(), last, 0u);
+                and_bits32(H.split(), f, 0u);
+                dact += (int32_t)nv - 1;
                 if (!was_active) {
-                    const unsigned long long bf = 1ull << (f & 63), bm = 1ull << (max_first & 63);
-                    if (!(atomicOr((unsigned long long*)&H.active()[f >> 6], bf) & bf)) ++dact;
-                    if (atomicAnd((unsigned long long*)&H.active()[max_first >> 6], ~bm) & bm) --dact;
+                    or_bits32(H.active(), f, 1u);
+                    and_bits32(H.active(), max_first, 0u);
                 }
             }
             // the last candidate in position order wins
