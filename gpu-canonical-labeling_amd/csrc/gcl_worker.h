@@ -1357,16 +1357,20 @@ struct Worker {
     // in position order, and the cell masks of the new fragments.
     __device__ void general_tail(Ctx& X, uint32_t s, uint32_t& longcode, int& hint) {
         const int l = lane_id();
-        // compact list of the pre-step non-singleton cells (position order)
+        // compact list of the pre-step non-singleton cells (position order),
+        // a lane per 32-bit half of an nsmask word (fewer bits per lane than
+        // whole words, and 32-bit bit scans)
         uint32_t nns = 0;
-        for (uint32_t base = 0; base < X.nw; base += kWave) {
+        const uint32_t nh = 2 * X.nw;
+        const uint32_t* half = (const uint32_t*)H.nsmask();
+        for (uint32_t base = 0; base < nh; base += kWave) {
             const uint32_t w = base + (uint32_t)l;
-            uint64_t bits = w < X.nw ? H.nsmask()[w] : 0ull;
-            const uint32_t c0 = (uint32_t)__popcll(bits);
+            uint32_t bits = w < nh ? half[w] : 0u;
+            const uint32_t c0 = (uint32_t)__popc(bits);
             const uint32_t inc = wave_scan(c0);
             uint32_t j = nns + inc - c0;
             while (bits) {
-                H.sb(j++) = (ix_t)(w * 64u + (uint32_t)__builtin_ctzll(bits));
+                H.sb(j++) = (ix_t)(w * 32u + (uint32_t)__builtin_ctz(bits));
                 bits &= bits - 1;
             }
             nns += readlane(inc, kWave - 1);
