@@ -1705,50 +1705,67 @@ struct Worker {
         const uint64_t bud0 = uni(budget);
         const uint32_t bud32 = bud0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bud0;
         uint32_t nst = 0;  // steps of this refinement
-        while (X.ncells < X.n && X.nactive > 0) {
-            if (nst == bud32) {
-                status = GCL_E_INTERNAL;  // step budget exhausted
-                break;
+        // Singleton splitters of degree <= 64 run back to back in the inner
+        // loop, and a larger splitter leaves it for one general step: the
+        // values that the singleton steps carry then stay in the same
+        // registers from one step to the next.
+        bool more = true;
+        while (more) {
+            uint32_t p = 0, e = 0;
+            for (;;) {
+                if (!(X.ncells < X.n && X.nactive > 0)) {
+                    more = false;
+                    break;
+                }
+                if (nst == bud32) {
+                    status = GCL_E_INTERNAL;  // step budget exhausted
+                    more = false;
+                    break;
+                }
+                // splitter: the first active position at or after the hint, else
+                // the first one; the hint's own word is tested first (one scalar load)
+                const uint32_t hw = (uint32_t)hint >> 6;
+                const uint64_t hword = uni(H.active()[hw]);
+                const uint64_t near = hword & (~0ull << ((uint32_t)hint & 63u));
+                int32_t sp;
+                uint64_t word = hword;  // the active word holding the splitter
+                if (near) {
+                    sp = (int32_t)(hw * 64u + (uint32_t)__builtin_ctzll(near));
+                } else {
+                    sp = first_set_from(H.active(), (hw + 1) * 64u, X.nw, &word);
+                    if (sp < 0) sp = first_set_from(H.active(), 0, X.nw, &word);
+                }
+                p = uni((uint32_t)sp);
+                // clear its bit from the word in hand (no other lane writes the
+                // bitmap between steps)
+                if (l == 0) H.active()[p >> 6] = uni(word) & ~(1ull << (p & 63u));
+                --X.nactive;
+                ++nst;
+                // the cell end and the splitter's vertex (used when the cell is a
+                // singleton) are independent loads
+                e = uni(cend(p));
+                const uint32_t wp = uni((uint32_t)H.elem(p));
+                // a singleton splitter's neighbour ids are requested before the
+                // sync, so that the load overlaps it and the bookkeeping
+                uint32_t w = 0, e0 = 0, deg = 0, u = 0;
+                if (e == p) {
+                    w = wp;
+                    e0 = uni(xrow(X, w));
+                    deg = uni(xrow(X, w + 1)) - e0;
+                    if (deg <= (uint32_t)kWave && (uint32_t)l < deg) u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
+                }
+                longcode = GCL_MASH(longcode, p + e);
+                if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
+                H.sync();
+                if (e != p || deg > (uint32_t)kWave) break;
+                singleton_step(X, u, deg, longcode, hint);
+                // (an LDS class has n <= 8192, so a lane's count stays below
+                // n^2 < 2^32 over a whole refinement)
+                if constexpr (!Hot::kLds)
+                    if ((nst & 0x3FFFu) == 0) flush_split_bytes(X);
             }
-            // splitter: the first active position at or after the hint, else
-            // the first one; the hint's own word is tested first (one scalar load)
-            const uint32_t hw = (uint32_t)hint >> 6;
-            const uint64_t hword = uni(H.active()[hw]);
-            const uint64_t near = hword & (~0ull << ((uint32_t)hint & 63u));
-            int32_t sp;
-            uint64_t word = hword;  // the active word holding the splitter
-            if (near) {
-                sp = (int32_t)(hw * 64u + (uint32_t)__builtin_ctzll(near));
-            } else {
-                sp = first_set_from(H.active(), (hw + 1) * 64u, X.nw, &word);
-                if (sp < 0) sp = first_set_from(H.active(), 0, X.nw, &word);
-            }
-            const uint32_t p = uni((uint32_t)sp);
-            // clear its bit from the word in hand (no other lane writes the
-            // bitmap between steps)
-            if (l == 0) H.active()[p >> 6] = uni(word) & ~(1ull << (p & 63u));
-            --X.nactive;
-            ++nst;
-            // the cell end and the splitter's vertex (used when the cell is a
-            // singleton) are independent loads
-            const uint32_t e = uni(cend(p));
-            const uint32_t wp = uni((uint32_t)H.elem(p));
-            // a singleton splitter's neighbour ids are requested before the
-            // sync, so that the load overlaps it and the bookkeeping
-            uint32_t w = 0, e0 = 0, deg = 0, u = 0;
-            if (e == p) {
-                w = wp;
-                e0 = uni(xrow(X, w));
-                deg = uni(xrow(X, w + 1)) - e0;
-                if (deg <= (uint32_t)kWave && (uint32_t)l < deg) u = (uint32_t)gload(X.col + e0 + (uint32_t)l);
-            }
-            longcode = GCL_MASH(longcode, p + e);
-            if (e > p) longcode = GCL_MASH(longcode, e - p + 1);
-            H.sync();
-            if (e == p && deg <= (uint32_t)kWave) singleton_step(X, u, deg, longcode, hint);
-            else general_step(X, p, e, longcode, hint);
-            // (an LDS class has n <= 8192, so a lane's count stays below
-            // n^2 < 2^32 over a whole refinement)
+            if (!more) break;
+            general_step(X, p, e, longcode, hint);
             if constexpr (!Hot::kLds)
                 if ((nst & 0x3FFFu) == 0) flush_split_bytes(X);
         }
