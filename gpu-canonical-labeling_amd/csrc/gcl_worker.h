@@ -994,16 +994,17 @@ struct Worker {
     template <uint32_t N>  // network size: a power of two >= the cell size
     __device__ __forceinline__ uint32_t regroup_small(uint32_t f, uint32_t c, uint32_t s) {
         const uint32_t L = c - f + 1;
+        // every slot loads (slots past the cell repeat its last member and
+        // take the sentinel key): no divergent branches, and the loads of
+        // all slots are in flight together
         uint32_t kk[N];
 #pragma unroll
+        for (uint32_t t = 0; t < N; ++t) kk[t] = H.elem(f + min(t, L - 1));
+#pragma unroll
         for (uint32_t t = 0; t < N; ++t) {
-            if (t < L) {
-                const uint32_t v = H.elem(f + t);
-                const uint32_t kv = (s == 1) ? 1u - H.cnt_ld(v) : H.cnt_ld(v);
-                kk[t] = (kv << 16) | v;
-            } else {
-                kk[t] = 0xFFFFFFFFu;
-            }
+            const uint32_t cv = H.cnt_ld(kk[t]);
+            const uint32_t kv = (s == 1) ? 1u - cv : cv;
+            kk[t] = t < L ? ((kv << 16) | kk[t]) : 0xFFFFFFFFu;
         }
         // bitonic network; each compare-exchange is a min and a max
 #pragma unroll
@@ -1022,31 +1023,31 @@ struct Worker {
                 }
             }
         }
-        uint32_t kmin = kk[0] >> 16, kmax = 0;
+        // each slot's fragment: its first (fs) and last (ls) slot (a key
+        // never equals the sentinel's, so fragments end at the cell's end)
+        uint32_t fs[N], ls[N];
+        fs[0] = 0;
 #pragma unroll
-        for (uint32_t t = 0; t < N; ++t)
-            if (t + 1 == L) kmax = kk[t] >> 16;
-        if (kmin == kmax) return 1;
-        uint32_t nfrag = 0, end = 0;
+        for (uint32_t t = 1; t < N; ++t) fs[t] = (kk[t] >> 16) == (kk[t - 1] >> 16) ? fs[t - 1] : t;
+        ls[N - 1] = N - 1;
 #pragma unroll
-        for (int t = (int)N - 1; t >= 0; --t) {
-            if ((uint32_t)t < L) {
-                const uint32_t key = kk[t] >> 16, v = kk[t] & 0xFFFFu;
-                if ((uint32_t)t + 1 == L || (kk[t + 1 < (int)N ? t + 1 : t] >> 16) != key) end = (uint32_t)t;
-                H.elem(f + (uint32_t)t) = (ix_t)v;
-                H.pos(v) = (ix_t)(f + (uint32_t)t);
-                // (t descends: the fragment's first position, reached last,
-                // overwrites the entry of its last one)
-                H.ce(f + (uint32_t)t) = (ix_t)(f + end);
-                if (t == 0 || (kk[t > 0 ? t - 1 : 0] >> 16) != key) {
-                    H.ce(f + end) = (ix_t)(f + (uint32_t)t);
-                    if (f + end != c) set_bit_atomic(H.lasts(), f + end);
-                    ++nfrag;
-                }
+        for (int t = (int)N - 2; t >= 0; --t) ls[t] = (kk[t] >> 16) == (kk[t + 1] >> 16) ? ls[t + 1] : (uint32_t)t;
+        uint32_t nfrag = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < N; ++t) nfrag += (t < L && fs[t] == t) ? 1u : 0u;
+        if (nfrag == 1) return 1;  // one key: nothing moves
+#pragma unroll
+        for (uint32_t t = 0; t < N; ++t) {
+            if (t < L) {
+                const uint32_t v = kk[t] & 0xFFFFu, p = f + t;
+                H.elem(p) = (ix_t)v;
+                H.pos(v) = (ix_t)p;
+                H.ce(p) = (ix_t)(f + (ls[t] == t ? fs[t] : ls[t]));
+                or_bits32(H.lasts(), p, (ls[t] == t && p != c) ? 1u : 0u);
             }
         }
-        set_bit_atomic(H.split(), f);
-        set_bit_atomic(H.oldend(), c);
+        or_bits32(H.split(), f, 1u);
+        or_bits32(H.oldend(), c, 1u);
         return nfrag;
     }
 
