@@ -639,20 +639,29 @@ struct Worker {
     LevelRec* lv;
     int32_t *firstcode, *canon_code;
     uint32_t *first_keys, *best_keys, *first_row, *best_row, *subgroup;
-    uint64_t arena_cap;
-    uint32_t ring, levels;
-    uint64_t step_budget;
+    // the launch parameters (constant over the launch): the ring size,
+    // snapshot levels, arena capacity, step budget and the speculation
+    // records' layouts are read from them where used rather than kept here
+    const KParams* kp;
+    __device__ __forceinline__ uint32_t ring() const { return gload(&kp->ring); }
+    __device__ __forceinline__ uint32_t levels() const { return gload(&kp->levels); }
+    __device__ __forceinline__ uint64_t arena_cap() const { return gload(&kp->arena_cap); }
+    __device__ __forceinline__ uint64_t step_budget() const { return gload(&kp->step_budget); }
+    __device__ __forceinline__ SpecLayout sl() const {
+        return make_spec_layout(gload(&kp->maxn), gload(&kp->spec_slots));
+    }
+    __device__ __forceinline__ DescLayout dly() const {
+        return make_desc_layout(gload(&kp->maxn), gload(&kp->dspec_items), gload(&kp->dspec_depth),
+                                gload(&kp->dspec_path));
+    }
     uint64_t life;      // wall-clock tick at which this worker's loop ends
     uint64_t deadline;  // tick at which the current search gives up (<= life)
     // search state (reference src/sequential.h:24-58)
     uint32_t level, seq_size, tc_level_reached, eqlevfirst, first_seq_size, best_seq_size;
     int eqlevcanon, comp_canon, canon_level, code;
-    bool first_discovered;
     int64_t num_aut;
     uint64_t nodes, steps, csr_bytes, part_bytes, budget;
     int32_t status;
-    // speculation (owner side)
-    bool root_changed;       // the partition held at level 1 is not the root's (own helping)
     // diagnostics: totals of this worker over the launch (DiagRecord)
     uint64_t dg_nodes, dg_steps, dg_helped, dg_search, dg_wait, dg_help, dg_idle;
     // phase times of the own search (refine, classify, orbits, backtracking
@@ -660,7 +669,6 @@ struct Worker {
     // rejected by their code; kept only with diagnostics on
     uint64_t dg_refine, dg_classify, dg_orbits, dg_backtrack;
     uint32_t dg_adopted, dg_rejected;
-    bool dg_on;
     __device__ __forceinline__ uint64_t dg_t0() const { return dg_on ? wall_clock64() : 0ull; }
     __device__ __forceinline__ void dg_add(uint64_t& acc, uint64_t t0) {
         if (dg_on) acc += wall_clock64() - t0;
@@ -672,13 +680,15 @@ struct Worker {
     // (-1: none), whose claimed items may still be in progress
     int32_t d_slot, d_old;
     uint32_t d_epoch, d_old_epoch;
-    bool d_renew;  // a new best leaf: publish its path's descents
     uint8_t* d_rec;
-    DescLayout DLy;
-    bool spec_pending;       // counted in *pending until it publishes or ends
     uint32_t spec_epoch, spec_cursor, spec_count;
     uint8_t* spec_rec;
-    SpecLayout SL;
+    // flags (together, to keep the object small: it lives in LDS)
+    bool first_discovered;
+    bool root_changed;   // the partition held at level 1 is not the root's (own helping)
+    bool dg_on;
+    bool d_renew;        // a new best leaf: publish its path's descents
+    bool spec_pending;   // counted in *pending until it publishes or ends
 
     // ---------------------------------------------------------- cell array
     // One 16-bit entry per position, ce[p]: the last position of p's cell
@@ -829,7 +839,7 @@ struct Worker {
 
     __device__ void snapshot(uint32_t lev) {
         // cell boundaries of the level-`lev` node, for backtracking
-        if (lev >= levels) {
+        if (lev >= levels()) {
             status = GCL_E_CAPACITY;
             return;
         }
@@ -1832,7 +1842,7 @@ struct Worker {
         const uint32_t t = c - (uint32_t)f + 1;
         const uint32_t idx = tc_level_reached;
         const uint64_t off = idx == 0 ? 0 : (uint64_t)lv[idx - 1].arena_off + lv[idx - 1].tc_len;
-        if (off + t > arena_cap) {
+        if (off + t > arena_cap()) {
             status = GCL_E_CAPACITY;
             return false;
         }
@@ -1872,7 +1882,7 @@ struct Worker {
     __device__ void compute_orbits(uint32_t li) {
         const int l = lane_id();
         const LevelRec r = lv[li];
-        const int64_t upto = min(num_aut, (int64_t)ring);
+        const int64_t upto = min(num_aut, (int64_t)ring());
         const int64_t from = r.last_num_aut < 0 ? 0 : r.last_num_aut;
         // ring slots whose automorphism fixes the sequence: one slot per lane;
         // aut_fix[i] = 2k+1 when seq[k] is the first entry the slot moves,
@@ -2129,11 +2139,11 @@ struct Worker {
 
     __device__ void bind_auts(uint8_t* region) {
         aut_fix = (uint32_t*)region;
-        auts = region ? (ix_t*)(region + aut_fix_bytes(ring)) : nullptr;
+        auts = region ? (ix_t*)(region + aut_fix_bytes(ring())) : nullptr;
     }
 
     __device__ void store_aut(const ix_t* ref_label) {
-        const uint64_t slot = (uint64_t)(num_aut % (int64_t)ring);
+        const uint64_t slot = (uint64_t)(num_aut % (int64_t)ring());
         ix_t* dst = auts + slot * n;
         for (uint32_t j = (uint32_t)lane_id(); j < n; j += kWave) dst[ref_label[j]] = H.elem(j);
         if (lane_id() == 0) aut_fix[slot] = 0;  // nothing of seq checked yet
@@ -2147,7 +2157,7 @@ struct Worker {
     // lengths beyond it, or a mismatch at an entry since rewritten, go back
     // to "matches the first fix_low entries, rest unchecked".
     __device__ void truncate_fix() {
-        const int64_t cnt = min(num_aut, (int64_t)ring);
+        const int64_t cnt = min(num_aut, (int64_t)ring());
         const uint32_t low = fix_low;
         fix_low = 0xFFFFFFFFu;
         if (cnt == 0 || low == 0xFFFFFFFFu) return;
@@ -2304,8 +2314,8 @@ struct Worker {
     }
 
     // ------------------------------------------------- root speculation
-    __device__ SpecHeader* spec_header() { return (SpecHeader*)(spec_rec + SL.header); }
-    __device__ uint32_t* spec_state() { return (uint32_t*)(spec_rec + SL.state); }
+    __device__ SpecHeader* spec_header() { return (SpecHeader*)(spec_rec + sl().header); }
+    __device__ uint32_t* spec_state() { return (uint32_t*)(spec_rec + sl().state); }
 
     // This graph will not publish (any more): it leaves the launch's count of
     // graphs that may, which lets helpers leave once nothing is published.
@@ -2352,20 +2362,19 @@ struct Worker {
         }
         spec_slot = slot;
         spec_rec = P.spec + (uint64_t)slot * P.spec_stride;
-        SL = make_spec_layout(P.maxn, P.spec_slots);
         SpecHeader* hd = spec_header();
         const uint32_t epoch = hd->epoch + 1;
         spec_epoch = epoch;
-        const uint32_t count = min(cnt, spec_item_cap(SL, P.spec_stride, n));
+        const uint32_t count = min(cnt, spec_item_cap(sl(), P.spec_stride, n));
         spec_count = count;
         spec_cursor = 0;
         // root partition and item list
-        ix_t* re = (ix_t*)(spec_rec + SL.root_elem);
-        uint64_t* rl = (uint64_t*)(spec_rec + SL.root_lasts);
+        ix_t* re = (ix_t*)(spec_rec + sl().root_elem);
+        uint64_t* rl = (uint64_t*)(spec_rec + sl().root_lasts);
         const uint64_t* s1 = snap + 1ull * (nw + 1);
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) re[p] = H.elem(p);
         for (uint32_t w = (uint32_t)lane_id(); w < nw; w += kWave) rl[w] = s1[w];
-        ix_t* vert = (ix_t*)(spec_rec + SL.vert);
+        ix_t* vert = (ix_t*)(spec_rec + sl().vert);
         uint32_t run = 0;
         for (uint32_t base = r.cursor; base < r.tc_len && run < count; base += kWave) {
             const uint32_t i = base + (uint32_t)lane_id();
@@ -2410,7 +2419,7 @@ struct Worker {
     static constexpr int kNotTaken = 0, kTaken = 1, kRejected = 2;
     __device__ int spec_take(const KParams& P, uint32_t v) {
         if (spec_slot < 0) return kNotTaken;
-        const ix_t* vert = (const ix_t*)(spec_rec + SL.vert);
+        const ix_t* vert = (const ix_t*)(spec_rec + sl().vert);
         uint32_t* state = spec_state();
         const uint32_t E = spec_epoch << 2;
         // skip (and withdraw) published items that became non-representatives
@@ -2439,13 +2448,13 @@ struct Worker {
         }
         dg_wait += wall_clock64() - tw;
         agent_acquire();
-        const uint32_t hw = uni(((const uint32_t*)(spec_rec + SL.code))[j]);
+        const uint32_t hw = uni(((const uint32_t*)(spec_rec + sl().code))[j]);
         if (hw == kFailedCode) {
             status = GCL_E_INTERNAL;  // the helper's refinement failed
             return kNotTaken;
         }
         const uint32_t hc = hw & ~kCodeOnly;
-        const uint64_t* cnt_rec = (const uint64_t*)(spec_rec + SL.steps) + 4ull * j;
+        const uint64_t* cnt_rec = (const uint64_t*)(spec_rec + sl().steps) + 4ull * j;
         // A child whose code is below the best path's at this level and
         // differs from the first path's is classified kBad before its
         // partition is looked at (classify_node, reference
@@ -2470,7 +2479,7 @@ struct Worker {
         // rejects; should this worker not be in the state that rejects it,
         // the child is refined here
         if (hw & kCodeOnly) return kNotTaken;
-        const uint8_t* item = spec_rec + SL.payload + (uint64_t)j * spec_item_bytes(n);
+        const uint8_t* item = spec_rec + sl().payload + (uint64_t)j * spec_item_bytes(n);
         const ix_t* ie = (const ix_t*)item;
         const uint64_t* il = (const uint64_t*)(item + spec_item_lasts(n));
         for (uint32_t p = (uint32_t)lane_id(); p < n; p += kWave) H.elem(p) = ie[p];
@@ -2541,8 +2550,8 @@ struct Worker {
     }
 
     // ---------------------------------------------- descent speculation
-    __device__ DescHeader* desc_header() { return (DescHeader*)(d_rec + DLy.header); }
-    __device__ uint32_t* desc_state() { return (uint32_t*)(d_rec + DLy.state); }
+    __device__ DescHeader* desc_header() { return (DescHeader*)(d_rec + dly().header); }
+    __device__ uint32_t* desc_state() { return (uint32_t*)(d_rec + dly().state); }
 
     // At a new best leaf of level d (the first leaf is the first of them):
     // lv[0 .. d-2] are the best path's target cells, each cursor just after
@@ -2576,13 +2585,12 @@ struct Worker {
         slot = __shfl(slot, 0, kWave);
         if (slot < 0) return;
         d_rec = P.dspec + (uint64_t)slot * P.dspec_stride;
-        DLy = make_desc_layout(P.maxn, P.dspec_items, P.dspec_depth, P.dspec_path);
         const uint64_t nw1 = (uint64_t)P.maxn / 64 + 2;
         uint8_t* rec = d_rec;
-        uint32_t* lfirst = (uint32_t*)(rec + DLy.level_first);
-        uint32_t* lcount = (uint32_t*)(rec + DLy.level_count);
-        ix_t* ilev = (ix_t*)(rec + DLy.item_level);
-        ix_t* ivert = (ix_t*)(rec + DLy.item_vert);
+        uint32_t* lfirst = (uint32_t*)(rec + dly().level_first);
+        uint32_t* lcount = (uint32_t*)(rec + dly().level_count);
+        ix_t* ilev = (ix_t*)(rec + dly().item_level);
+        ix_t* ivert = (ix_t*)(rec + dly().item_vert);
         // items, deepest level first: lane q of a chunk takes level lmax - q
         uint32_t count = 0;
         for (uint32_t base = 0; base + 2 <= lmax; base += kWave) {
@@ -2597,7 +2605,7 @@ struct Worker {
             }
             const uint32_t inc = wave_scan(c);
             const uint32_t off = count + inc - c;
-            const uint32_t take = valid ? (off >= DLy.items ? 0u : min(c, DLy.items - off)) : 0u;
+            const uint32_t take = valid ? (off >= dly().items ? 0u : min(c, dly().items - off)) : 0u;
             if (valid) {
                 lfirst[L] = off;
                 lcount[L] = take;
@@ -2606,7 +2614,7 @@ struct Worker {
                     ivert[off + i] = arena_mem[r.arena_off + r.cursor + i];
                 }
             }
-            count = min(count + readlane(inc, kWave - 1), DLy.items);
+            count = min(count + readlane(inc, kWave - 1), dly().items);
         }
         count = uni(count);
         if (count == 0) {
@@ -2620,9 +2628,9 @@ struct Worker {
             lcount[l] = 0;
         }
         // the first leaf's element vector and the first path's snapshots
-        ix_t* re = (ix_t*)(rec + DLy.elem);
+        ix_t* re = (ix_t*)(rec + dly().elem);
         for (uint32_t p = (uint32_t)l; p < n; p += kWave) re[p] = H.elem(p);
-        uint64_t* rs = (uint64_t*)(rec + DLy.snaps);
+        uint64_t* rs = (uint64_t*)(rec + dly().snaps);
         for (uint32_t L = 2; L <= lmax; ++L)
             for (uint32_t w = (uint32_t)l; w < nw; w += kWave) rs[(uint64_t)L * nw1 + w] = snap[(uint64_t)L * (nw + 1) + w];
         DescHeader* hd = desc_header();
@@ -2656,9 +2664,9 @@ struct Worker {
     // the deadline stays out of it for the rest of the launch).
     __device__ void desc_end(const KParams& P, int32_t slot, uint32_t epoch, bool wait) {
         uint8_t* rec = P.dspec + (uint64_t)slot * P.dspec_stride;
-        uint32_t* state = (uint32_t*)(rec + DLy.state);
+        uint32_t* state = (uint32_t*)(rec + dly().state);
         const uint32_t E = epoch << 2;
-        const uint32_t count = uni(((const DescHeader*)(rec + DLy.header))->count);
+        const uint32_t count = uni(((const DescHeader*)(rec + dly().header))->count);
         if (lane_id() == 0) atomicAnd(&P.dboard[slot >> 6], ~(unsigned long long)(1ull << (slot & 63)));
         group_sync();
         bool held = false;
@@ -2706,7 +2714,7 @@ struct Worker {
     // continues from the state the last classification left.
     __device__ int desc_take(const KParams& P, uint32_t v) {
         const uint32_t L = level;
-        if (L < 2 || L > DLy.path || L > uni(desc_header()->levels)) return kNotTaken;
+        if (L < 2 || L > dly().path || L > uni(desc_header()->levels)) return kNotTaken;
         // on the best path, whose descents the record holds: seq[0 .. L-2]
         // equals best_seq[0 .. L-2]
         if (L - 1 > best_seq_size) return kNotTaken;
@@ -2714,9 +2722,9 @@ struct Worker {
             const uint32_t i = base + (uint32_t)lane_id();
             if (ballot(i < L - 1 && seq[i] != best_seq[i])) return kNotTaken;
         }
-        const uint32_t lf = uni(((const uint32_t*)(d_rec + DLy.level_first))[L]);
-        const uint32_t lc = uni(((const uint32_t*)(d_rec + DLy.level_count))[L]);
-        const ix_t* ivert = (const ix_t*)(d_rec + DLy.item_vert);
+        const uint32_t lf = uni(((const uint32_t*)(d_rec + dly().level_first))[L]);
+        const uint32_t lc = uni(((const uint32_t*)(d_rec + dly().level_count))[L]);
+        const ix_t* ivert = (const ix_t*)(d_rec + dly().item_vert);
         uint32_t j = 0xFFFFFFFFu;
         for (uint32_t i = 0; i < lc; ++i)
             if ((uint32_t)ivert[lf + i] == v) {
@@ -2740,11 +2748,11 @@ struct Worker {
         }
         dg_wait += wall_clock64() - tw;
         agent_acquire();
-        const uint32_t nr = uni(((const uint32_t*)(d_rec + DLy.nrec))[j]);
+        const uint32_t nr = uni(((const uint32_t*)(d_rec + dly().nrec))[j]);
         const uint32_t nrec = nr & 0xFFFFu;
         if ((nr & kDescFailed) || nrec == 0) return kNotTaken;
         ++dg_adopted;
-        desc_replay(P, d_rec + DLy.payload + (uint64_t)j * DLy.item_bytes, nrec);
+        desc_replay(P, d_rec + dly().payload + (uint64_t)j * dly().item_bytes, nrec);
         return kTaken;
     }
 
@@ -2754,8 +2762,8 @@ struct Worker {
     __device__ void desc_replay(const KParams& P, const uint8_t* item, uint32_t nrec) {
         const int l = lane_id();
         const DescLevel* R = (const DescLevel*)item;
-        const uint64_t* LS = (const uint64_t*)(item + DLy.lasts_off);
-        const ix_t* FE = (const ix_t*)(item + DLy.elem_off);
+        const uint64_t* LS = (const uint64_t*)(item + dly().lasts_off);
+        const ix_t* FE = (const ix_t*)(item + dly().elem_off);
         const uint64_t nw1 = (uint64_t)P.maxn / 64 + 2;
         const uint32_t L0 = level;
         // the element vector of the last recorded level refines every level
@@ -2781,7 +2789,7 @@ struct Worker {
             }
             budget -= sk;
             ++level;
-            if (level >= levels) {
+            if (level >= levels()) {
                 status = GCL_E_CAPACITY;
                 return;
             }
@@ -2816,7 +2824,7 @@ struct Worker {
                 return;
             }
             const uint64_t off = idx == 0 ? 0 : (uint64_t)lv[idx - 1].arena_off + lv[idx - 1].tc_len;
-            if (off + tl > arena_cap) {
+            if (off + tl > arena_cap()) {
                 status = GCL_E_CAPACITY;
                 return;
             }
@@ -2882,7 +2890,7 @@ struct Worker {
         DescLevel* R = (DescLevel*)item;
         uint64_t* LS = (uint64_t*)(item + L.lasts_off);
         ix_t* FE = (ix_t*)(item + L.elem_off);
-        budget = step_budget;
+        budget = step_budget();
         status = 0;
         // level lev's target cell: the first non-singleton cell
         int32_t f = first_set_from(H.nsmask(), 0, nw);
@@ -3052,7 +3060,7 @@ struct Worker {
         code = 0;
         first_seq_size = best_seq_size = 0;
         first_discovered = false;
-        budget = step_budget;
+        budget = step_budget();
         spec_slot = -1;
         d_slot = d_old = -1;
         d_renew = false;
@@ -3123,7 +3131,7 @@ struct Worker {
         rebuild_from_lasts();
         const uint32_t v = ((const ix_t*)(rec + L.vert))[j];
         const uint64_t s0 = steps, b0 = csr_bytes, pb0 = part_bytes;
-        budget = step_budget;
+        budget = step_budget();
         status = 0;
         individualize_and_refine(v, tcf);
         // The owner classifies a root child whose code is below the best
@@ -3286,6 +3294,7 @@ __device__ __forceinline__ int help_counted(Worker<Hot>& W, const KParams& P) {
 
 template <class Hot>
 __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, const Layout& L) {
+    W.kp = &P;
     W.snap = (uint64_t*)(base + L.snap);
     W.seq = (ix_t*)(base + L.seq);
     W.first_seq = (ix_t*)(base + L.first_seq);
@@ -3302,12 +3311,8 @@ __device__ void bind_scratch(Worker<Hot>& W, const KParams& P, uint8_t* base, co
     W.first_row = (uint32_t*)(base + L.first_row);
     W.best_row = (uint32_t*)(base + L.best_row);
     W.subgroup = (uint32_t*)(base + L.subgroup);
-    W.arena_cap = P.arena_cap;
-    W.ring = P.ring;
     W.bind_auts(P.aut_pool ? nullptr : base + L.auts);
     W.H.bind_lists(P.lists ? P.lists + (uint64_t)blockIdx.x * P.lists_stride : base + L.lists, P.maxn);
-    W.levels = L.levels;
-    W.step_budget = P.step_budget;
     const uint64_t now = wall_clock64();
     W.life = P.time_limit > ~0ull - now ? ~0ull : now + P.time_limit;
     W.deadline = W.life;
