@@ -1062,8 +1062,9 @@ struct Worker {
     }
 
     // Split of a large touched cell [f, c] under a splitter of more than one
-    // vertex when few of its members are touched (at most a wave, under a
-    // quarter of the cell).  Fragments go in increasing count, so the
+    // vertex when few of its members are touched (at most a wave, at most a
+    // third of the cell: the outsiders' positions and ids are staged in
+    // tmp[f + tc, f + 3tc)).  Fragments go in increasing count, so the
     // untouched members (count 0) stay a block at the front and the touched
     // ones form the tail [B, c], grouped by count: touched members outside
     // the tail swap with untouched ones inside it (ranks from ballots), the
@@ -1092,7 +1093,7 @@ struct Worker {
             tc += (uint32_t)__popcll(m);
         }
         tc = uni(tc);
-        if (tc == 0 || tc > (uint32_t)kWave || 4 * tc >= L) return false;
+        if (tc == 0 || tc > (uint32_t)kWave || 3 * tc > L) return false;
         H.sync();
         const bool act = (uint32_t)l < tc;
         const uint32_t u = act ? (uint32_t)H.tmp(f + (uint32_t)l) : 0u;
@@ -1103,7 +1104,7 @@ struct Worker {
     // lanes with `in` hold the cell's touched members u.
     __device__ bool sparse_split_lanes(Ctx& X, uint32_t f, uint32_t c, uint32_t L, bool in, uint32_t u) {
         const uint32_t tc = (uint32_t)__popcll(ballot(in));
-        if (tc == 0 || 4 * tc >= L) return false;
+        if (tc == 0 || 3 * tc > L) return false;
         sparse_split_core(X, f, c, L, tc, in, u);
         return true;
     }
