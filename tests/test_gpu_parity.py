@@ -110,6 +110,26 @@ def test_gpu_batch_matches_oracle_and_plain():
         assert np.array_equal(batch.labels_of(plain.labels, i), lab), i
 
 
+@pytest.mark.parametrize("lo,hi,count,seed", [(257, 512, 48, 22), (513, 1024, 36, 21)])
+def test_gpu_benchmark_mix_large_classes_match_oracle(lo, hi, count, seed):
+    """The benchmark mix's two largest LDS classes as a batch (G(n,p),
+    random regular and torus graphs, root children refined by helpers):
+    every graph's canonical form and search-tree node count equal the
+    oracle's, and the batch's algorithmic byte total is the oracle's sum."""
+    batch = gcl.mix_batch(count, seed, lo, hi)
+    _log(f"mix batch n in [{lo}, {hi}]: {batch.num_graphs} graphs")
+    res = gcl.batch_canon(batch, max_aut=10000, engine="gpu")
+    total = 0
+    for i in range(batch.num_graphs):
+        g = batch.graph(i)
+        o = oracle.canon_graph(g, max_aut=10000)
+        keys = gcl.canonical_keys(g, batch.labels_of(res.labels, i))
+        assert np.array_equal(keys, o["keys"]), (i, g.n, int(batch.kind[i]))
+        assert int(res.nodes[i]) == o["nodes"], (i, g.n, int(res.nodes[i]), o["nodes"])
+        total += o["alg_bytes"]
+    assert res.alg_bytes == total, (res.alg_bytes, total)
+
+
 def test_gpu_batch_coloured_mix_matches_oracle():
     # coloured graphs (an initial partition of several cells, vertices in
     # (colour, id) order) next to uncoloured ones in one batch, some large
