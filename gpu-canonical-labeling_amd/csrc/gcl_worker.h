@@ -604,10 +604,10 @@ struct WithDesc : Base {
 struct Ctx {
     uint32_t n, nw, ncells, nactive;
     uint64_t steps, csr_bytes;
-    // this refinement's singleton steps and the degrees of their splitters
-    // (32-bit: a refinement takes at most 2n steps), folded into csr_bytes
-    // at its end
-    uint32_t nsing, sdeg;
+    // this refinement's CSR words read by singleton steps, 3 + deg(w) per
+    // step (the splitter's id and row offsets, its neighbour ids): below
+    // 2n * 67 < 2^32, folded into csr_bytes at its end
+    uint32_t swords;
     // algorithmic bytes beyond the CSR reads: 8 per non-singleton cell hashed
     // under a non-singleton splitter, 8 per vertex of a cell that splits, 8n
     // per refinement (the element vector loaded and stored once)
@@ -888,8 +888,7 @@ struct Worker {
     // loaded by the caller
     __device__ __forceinline__ void singleton_step(Ctx& X, uint32_t u, uint32_t deg, uint32_t& longcode, int& hint) {
         const int l = lane_id();
-        ++X.nsing;
-        X.sdeg += deg;
+        X.swords += 3 + deg;
         uint32_t c = 0, f = 0, r = 0, q = 0;
         bool ns = false;
         if ((uint32_t)l < deg) {
@@ -1688,8 +1687,7 @@ struct Worker {
         X.nactive = uni(nactive);
         X.steps = uni(steps);
         X.csr_bytes = uni(csr_bytes);
-        X.nsing = 0;
-        X.sdeg = 0;
+        X.swords = 0;
         X.part_bytes = uni(part_bytes) + 8ull * X.n;
         X.lane_split = 0;
         X.col = uni(col);
@@ -1784,7 +1782,7 @@ struct Worker {
         flush_split_bytes(X);
         budget = bud0 - nst;
         X.steps += nst;
-        X.csr_bytes += 12ull * X.nsing + 4ull * X.sdeg;
+        X.csr_bytes += 4ull * X.swords;
         if (X.nactive) {
             for (uint32_t w = (uint32_t)l; w < X.nw; w += kWave) H.active()[w] = 0;
             X.nactive = 0;
