@@ -130,6 +130,29 @@ def test_gpu_benchmark_mix_large_classes_match_oracle(lo, hi, count, seed):
     assert res.alg_bytes == total, (res.alg_bytes, total)
 
 
+def test_gpu_bench_batch_sample_matches_oracle():
+    """The 65,536-graph batch that bench.py times, run whole on the device:
+    every 64th graph (1,024 of them, all sizes and families) has the
+    oracle's canonical form, search-tree node count and automorphism count."""
+    batch = gcl.mix_batch(65536, 1)
+    db = gcl.DeviceBatch(batch, max_aut=10000)
+    try:
+        _log("bench batch: one device step")
+        db.run()
+        res = db.results()
+        nodes, auts, _, _ = db.graph_stats()
+    finally:
+        db.close()
+    for i in range(0, batch.num_graphs, 64):
+        g = batch.graph(i)
+        o = oracle.canon_graph(g, max_aut=10000)
+        keys = gcl.canonical_keys(g, batch.labels_of(res.labels, i))
+        assert np.array_equal(keys, o["keys"]), (i, g.n, int(batch.kind[i]))
+        assert int(nodes[i]) == o["nodes"] and int(auts[i]) == o["automorphisms"], (i, g.n, int(nodes[i]), o["nodes"])
+        if i % 8192 == 0:
+            _log(f"checked through graph {i}")
+
+
 def test_gpu_batch_coloured_mix_matches_oracle():
     # coloured graphs (an initial partition of several cells, vertices in
     # (colour, id) order) next to uncoloured ones in one batch, some large
