@@ -1268,17 +1268,17 @@ struct Worker {
         for (uint32_t base = 0; base < nval; base += kWave) {
             const uint32_t x = base + (uint32_t)l < nval ? H.hits_ld(base + (uint32_t)l) : 0u;
             const uint32_t here = min((uint32_t)kWave, nval - base);
-            if (here == (uint32_t)kWave) {
+            // whole groups of 8 lanes with constant lane indices (one branch
+            // per group), then the last here % 8 values
+            const uint32_t full = here >> 3;
 #pragma unroll
-                for (int u = 0; u < kWave; ++u) lc = (lc ^ kx) + readlane(x, u);
-            } else {
-                uint32_t t = 0;
-                for (; t + 8 <= here; t += 8) {
+            for (uint32_t g = 0; g < 8; ++g) {
+                if (g < full) {
 #pragma unroll
-                    for (uint32_t u = 0; u < 8; ++u) lc = (lc ^ kx) + readlane(x, (int)(t + u));
+                    for (uint32_t u = 0; u < 8; ++u) lc = (lc ^ kx) + readlane(x, (int)(8 * g + u));
                 }
-                for (; t < here; ++t) lc = (lc ^ kx) + readlane(x, (int)t);
             }
+            for (uint32_t t = full * 8; t < here; ++t) lc = (lc ^ kx) + readlane(x, (int)t);
         }
         longcode = uni(lc) & 077777u;
         H.sync();
